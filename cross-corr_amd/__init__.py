@@ -1,0 +1,183 @@
+"""Python binding of libxcorr.so (the MI355X cross-correlation library) over its C ABI.
+
+The C ABI (include/xcorr.h) is the product boundary; this module only loads the shared
+object with ctypes and passes device pointers and the current HIP stream.  PyTorch is
+used for device memory and streams.  There is no CPU fallback: if the library or a GPU
+is missing, calls raise.
+
+The directory name contains a hyphen, so import it by path:
+
+    import importlib.util, pathlib
+    spec = importlib.util.spec_from_file_location("cross_corr_amd", "<repo>/cross-corr_amd/__init__.py")
+"""
+import ctypes
+import pathlib
+
+PKG_DIR = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libxcorr.so"
+
+ONE_TO_ONE, ONE_TO_MANY, N_TO_MN, N_TO_M = 0, 1, 2, 3
+FAMILIES = {"one_to_one": ONE_TO_ONE, "one_to_many": ONE_TO_MANY, "n_to_mn": N_TO_MN, "n_to_m": N_TO_M}
+
+PREC_AUTO, PREC_FP32, PREC_BF16X3, PREC_BF16, PREC_FP16, PREC_FFT64, PREC_FFT32 = range(7)
+PRECISIONS = {"auto": PREC_AUTO, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "bf16": PREC_BF16,
+              "fp16": PREC_FP16, "fft64": PREC_FFT64, "fft32": PREC_FFT32}
+
+STATUS = {0: "XC_OK", 1: "XC_ERR_INVALID_ARGUMENT", 2: "XC_ERR_HIP", 3: "XC_ERR_FFT",
+          4: "XC_ERR_UNSUPPORTED", 5: "XC_ERR_NO_MEMORY"}
+
+
+class XcError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_uint32), ("tile", ctypes.c_uint32), ("split", ctypes.c_uint32),
+                ("border", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+
+
+class Pair(ctypes.Structure):
+    _fields_ = [("left_off", ctypes.c_uint64), ("right_off", ctypes.c_uint64), ("out_off", ctypes.c_uint64),
+                ("rows", ctypes.c_uint32), ("cols", ctypes.c_uint32)]
+
+
+class LaunchDesc(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_char * 96), ("grid", ctypes.c_uint32 * 3), ("block", ctypes.c_uint32 * 3),
+                ("lds_bytes", ctypes.c_uint32), ("rows", ctypes.c_uint32), ("cols", ctypes.c_uint32),
+                ("pairs", ctypes.c_uint32), ("split", ctypes.c_uint32)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/xcorr.h.
+_vp, _u32, _u64, _i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+_st = ctypes.c_int
+SIGNATURES = {
+    "xc_cross_corr_f32": (_st, [_i32, _vp, _vp, _vp, _u32, _u32, _u32, _u32, ctypes.POINTER(Options), _vp]),
+    "xc_cross_corr_f64": (_st, [_i32, _vp, _vp, _vp, _u32, _u32, _u32, _u32, ctypes.POINTER(Options), _vp]),
+    "xc_num_results": (ctypes.c_int64, [_i32, _u32, _u32]),
+    "xc_batch_f32": (_st, [ctypes.POINTER(Pair), _u32, _vp, _vp, _vp, ctypes.POINTER(Options), _vp]),
+    "xc_scatter_f32": (_st, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _vp]),
+    "xc_scatter_f64": (_st, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _vp]),
+    "xc_hadamard_f32": (_st, [_vp, _vp, _u64, _u32, _vp]),
+    "xc_hadamard_f64": (_st, [_vp, _vp, _u64, _u32, _vp]),
+    "xc_hadamard_n_to_m_f32": (_st, [_vp, _vp, _vp, _u64, _u32, _u32, _vp]),
+    "xc_hadamard_n_to_m_f64": (_st, [_vp, _vp, _vp, _u64, _u32, _u32, _vp]),
+    "xc_normalize_fft_f32": (_st, [_vp, _vp, _u32, _u32, _u32, _vp]),
+    "xc_normalize_fft_f64": (_st, [_vp, _vp, _u32, _u32, _u32, _vp]),
+    "xc_fft_cross_corr_f32": (_st, [_i32, _vp, _vp, _vp, _u32, _u32, _u32, _u32, _i32, _i32, _vp]),
+    "xc_fft_cross_corr_f64": (_st, [_i32, _vp, _vp, _vp, _u32, _u32, _u32, _u32, _i32, _vp]),
+    "xc_run_ccn_shuffle_f32": (_st, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp]),
+    "xc_run_ccn_shuffle_one_to_many_multirow_both_multimat_right_f32":
+        (_st, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _vp]),
+    "xc_last_error": (ctypes.c_char_p, []),
+    "xc_last_launch": (ctypes.POINTER(LaunchDesc), []),
+    "xc_version": (ctypes.c_char_p, []),
+    "xc_device_info": (_st, [ctypes.c_char_p, ctypes.c_size_t]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Load libxcorr.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(status):
+    if status != 0:
+        raise XcError(status, lib().xc_last_error().decode())
+
+
+def last_launch():
+    d = lib().xc_last_launch().contents
+    return {"kernel": d.kernel.decode(), "grid": tuple(d.grid), "block": tuple(d.block), "lds_bytes": d.lds_bytes,
+            "rows": d.rows, "cols": d.cols, "pairs": d.pairs, "split": d.split}
+
+
+def _stream(stream):
+    if stream is not None:
+        return ctypes.c_void_p(stream)
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def result_shape(family, rows, cols, n, m):
+    fam = FAMILIES[family] if isinstance(family, str) else family
+    k = lib().xc_num_results(fam, n, m)
+    if k < 0:
+        raise XcError(1, "invalid matrix counts for this family")
+    return (k, 2 * rows - 1, 2 * cols - 1)
+
+
+def cross_corr(family, left, right, out=None, precision="auto", split=0, border=0, tile=0, stream=None):
+    """Batched full cross-correlation of torch device tensors.
+
+    left: (n, rows, cols), right: (m, rows, cols), float32 or float64, contiguous, on the GPU.
+    Returns `out` of shape (k, 2*rows-1, 2*cols-1) in the family's result order.
+    """
+    import torch
+    fam = FAMILIES[family] if isinstance(family, str) else family
+    if left.dim() == 2:
+        left = left.unsqueeze(0)
+    if right.dim() == 2:
+        right = right.unsqueeze(0)
+    if not (left.is_cuda and right.is_cuda):
+        raise XcError(1, "inputs must be GPU tensors")
+    if left.dtype != right.dtype or left.dtype not in (torch.float32, torch.float64):
+        raise XcError(1, "inputs must both be float32 or both float64")
+    left, right = left.contiguous(), right.contiguous()
+    n, rows, cols = left.shape
+    m = right.shape[0]
+    if tuple(right.shape[1:]) != (rows, cols):
+        raise XcError(1, "left and right matrices must have the same size")
+    shape = result_shape(fam, rows, cols, n, m)
+    if out is None:
+        out = torch.empty(shape, dtype=left.dtype, device=left.device)
+    opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, tile, split, border)
+    fn = lib().xc_cross_corr_f32 if left.dtype == torch.float32 else lib().xc_cross_corr_f64
+    check(fn(fam, _ptr(left), _ptr(right), _ptr(out), rows, cols, n, m, ctypes.byref(opt), _stream(stream)))
+    return out
+
+
+def fft_cross_corr(family, left, right, out=None, normalize=True, fp64_transform=False, stream=None):
+    """FFT method; normalize=False returns the unnormalised (2*rows) x (2*cols) circular results."""
+    import torch
+    fam = FAMILIES[family] if isinstance(family, str) else family
+    left, right = left.contiguous(), right.contiguous()
+    n, rows, cols = left.shape
+    m = right.shape[0]
+    k = result_shape(fam, rows, cols, n, m)[0]
+    shape = (k, 2 * rows - 1, 2 * cols - 1) if normalize else (k, 2 * rows, 2 * cols)
+    if out is None:
+        out = torch.empty(shape, dtype=left.dtype, device=left.device)
+    if left.dtype == torch.float32:
+        check(lib().xc_fft_cross_corr_f32(fam, _ptr(left), _ptr(right), _ptr(out), rows, cols, n, m,
+                                          int(normalize), int(fp64_transform), _stream(stream)))
+    else:
+        check(lib().xc_fft_cross_corr_f64(fam, _ptr(left), _ptr(right), _ptr(out), rows, cols, n, m,
+                                          int(normalize), _stream(stream)))
+    return out
+
+
+def batch(pairs, left_flat, right_flat, out_flat, precision="auto", stream=None):
+    """Uneven batch.  pairs: list of (left_off, right_off, out_off, rows, cols) in elements."""
+    arr = (Pair * len(pairs))(*[Pair(*p) for p in pairs])
+    opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, 0, 0, 0)
+    check(lib().xc_batch_f32(arr, len(pairs), _ptr(left_flat), _ptr(right_flat), _ptr(out_flat),
+                             ctypes.byref(opt), _stream(stream)))
+    return out_flat

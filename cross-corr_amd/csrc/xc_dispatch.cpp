@@ -1,0 +1,83 @@
+// Method dispatch and device workspace management.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "xc_methods.hpp"
+
+namespace xc {
+
+namespace {
+
+struct ws_slot {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+std::mutex g_ws_mutex;
+std::vector<std::vector<ws_slot>> g_ws;  // [device][slot]
+
+}  // namespace
+
+void* workspace(int slot, size_t bytes, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    int dev = 0;
+    XC_HIP(hipGetDevice(&dev));
+    if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
+    auto& v = g_ws[dev];
+    if ((int)v.size() <= slot) v.resize(slot + 1);
+    ws_slot& w = v[slot];
+    if (w.bytes < bytes) {
+        if (w.ptr) {
+            // The old buffer may still be in use by queued work on this stream.
+            XC_HIP(hipStreamSynchronize(s));
+            XC_HIP(hipFree(w.ptr));
+            w.ptr = nullptr;
+            w.bytes = 0;
+        }
+        if (hipMalloc(&w.ptr, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            fail(XC_ERR_NO_MEMORY, "workspace allocation of " + std::to_string(bytes) + " bytes failed");
+        }
+        w.bytes = bytes;
+    }
+    return w.ptr;
+}
+
+void run_family_f32(const family_args& a, const float* L, const float* R, float* O, const xc_options& opt,
+                    hipStream_t s) {
+    switch (opt.precision) {
+    case XC_PREC_AUTO:
+    case XC_PREC_FP32:
+        launch_direct_f32(a, L, R, O, opt.split, s);
+        return;
+    case XC_PREC_BF16X3:
+    case XC_PREC_BF16:
+    case XC_PREC_FP16:
+        launch_mfma_f32(a, L, R, O, (int)opt.precision, opt.border, s);
+        return;
+    case XC_PREC_FFT64:
+        run_fft_f32(a, L, R, O, true, true, s);
+        return;
+    case XC_PREC_FFT32:
+        run_fft_f32(a, L, R, O, true, false, s);
+        return;
+    default:
+        fail(XC_ERR_INVALID_ARGUMENT, "unknown precision " + std::to_string(opt.precision));
+    }
+}
+
+void launch_mfma_f32(const family_args&, const float*, const float*, float*, int, uint32_t, hipStream_t) {
+    fail(XC_ERR_UNSUPPORTED, "MFMA methods are not built yet");
+}
+
+void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, const float* R, float* O,
+                   const xc_options& opt, hipStream_t s) {
+    for (uint32_t i = 0; i < num_pairs; ++i) {
+        family_args a{XC_ONE_TO_ONE, pairs[i].rows, pairs[i].cols, 1, 1, 1};
+        run_family_f32(a, L + pairs[i].left_off, R + pairs[i].right_off, O + pairs[i].out_off, opt, s);
+    }
+}
+
+}  // namespace xc

@@ -1,0 +1,81 @@
+// Shared definitions of the HIP library: error plumbing, launch descriptors,
+// family -> pair mapping, XCD-aware block remapping and kernel entry declarations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/xcorr.h"
+
+namespace xc {
+
+// Error thrown inside the library and converted to an xc_status at the C ABI.
+struct error : std::runtime_error {
+    xc_status status;
+    error(xc_status s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+};
+
+[[noreturn]] inline void fail(xc_status s, const std::string& msg) { throw error(s, msg); }
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(XC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define XC_HIP(call) ::xc::hip_check((call), #call)
+
+// Records the launch in the calling thread's descriptor (src/kernel_args.cpp:7-55 keeps
+// the same facts for its failure messages).
+void record_launch(const char* kernel, dim3 grid, dim3 block, uint32_t lds, uint32_t rows, uint32_t cols,
+                   uint32_t pairs, uint32_t split);
+
+// k-th result of a family -> (left index, right index); see the table in xcorr.h.
+__host__ __device__ inline void pair_of(int family, uint32_t n, uint32_t m, uint32_t k, uint32_t& l,
+                                        uint32_t& r) {
+    switch (family) {
+    case XC_ONE_TO_MANY: l = 0; r = k; break;
+    case XC_N_TO_MN: l = k % n; r = k; break;
+    case XC_N_TO_M: l = k / m; r = k % m; break;
+    default: l = 0; r = 0; break;
+    }
+}
+
+inline int64_t num_results(int family, uint32_t n, uint32_t m) {
+    switch (family) {
+    case XC_ONE_TO_ONE: return (n == 1 && m == 1) ? 1 : -1;
+    case XC_ONE_TO_MANY: return (n == 1 && m >= 1) ? (int64_t)m : -1;
+    case XC_N_TO_MN: return (n >= 1 && m >= 1 && m % n == 0) ? (int64_t)m : -1;
+    case XC_N_TO_M: return (n >= 1 && m >= 1) ? (int64_t)n * m : -1;
+    default: return -1;
+    }
+}
+
+// The MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
+// with its own L2.  Remap so that each XCD receives a contiguous range of logical
+// block ids (neighbouring tiles of one pair share that XCD's L2).
+__device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
+    constexpr uint32_t XCD = 8;
+    uint32_t per = nblocks / XCD;
+    uint32_t full = per * XCD;
+    if (bid >= full) return bid;  // tail blocks keep their id
+    return (bid % XCD) * per + bid / XCD;
+}
+
+// Description of one launch over a regular family.
+struct family_args {
+    int family;
+    uint32_t rows, cols, n, m;
+    uint32_t pairs;  // number of results
+};
+
+// Kernel launchers (defined in the .hip files).
+void launch_direct_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t split,
+                       hipStream_t s);
+void launch_direct_f64(const family_args& a, const double* L, const double* R, double* O, uint32_t split,
+                       hipStream_t s);
+void launch_zero_f32(float* p, uint64_t count, hipStream_t s);
+void launch_zero_f64(double* p, uint64_t count, hipStream_t s);
+
+}  // namespace xc

@@ -1,0 +1,187 @@
+/*
+ * xcorr.h -- C ABI of the MI355X (gfx950) batched 2D full cross-correlation library
+ * (libxcorr.so, built from cross-corr_amd/csrc).
+ *
+ * Contract (src/validate.hpp:94-132 of the reference): for a left ("ref") matrix L
+ * and a right ("target") matrix R, both rows x cols, row-major,
+ *
+ *   out[(dy+rows-1)*(2*cols-1) + (dx+cols-1)] = sum_{r,c} L[r][c] * R[r-dy][c-dx]
+ *
+ * over the overlap, dy in [-(rows-1), rows-1], dx in [-(cols-1), cols-1].
+ * Matrices of one call are stacked back to back; results are (2*rows-1) x (2*cols-1).
+ *
+ * Conventions shared by every entry point:
+ *  - all matrix pointers are device pointers (hipMalloc'ed, or any memory the
+ *    GPU can read); the caller owns them; results are fully overwritten, the
+ *    caller never pre-zeroes anything;
+ *  - work is enqueued on `stream` (NULL = the null stream) and is asynchronous;
+ *  - no entry point throws: each returns an xc_status, and xc_last_error() gives
+ *    the message of the calling thread's last failure;
+ *  - arguments are validated on the host before any device work is enqueued.
+ */
+#ifndef XCORR_H
+#define XCORR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* xc_stream_t; /* the same type as hipStream_t */
+
+typedef enum xc_status {
+    XC_OK = 0,
+    XC_ERR_INVALID_ARGUMENT = 1, /* bad size, count, tuning value or family */
+    XC_ERR_HIP = 2,              /* a HIP runtime call failed */
+    XC_ERR_FFT = 3,              /* a hipFFT/rocFFT call failed */
+    XC_ERR_UNSUPPORTED = 4,      /* valid request this build cannot serve */
+    XC_ERR_NO_MEMORY = 5         /* workspace allocation failed */
+} xc_status;
+
+/* Families, with the ordering of result k (src/validate.hpp:216-298):
+ *  XC_ONE_TO_ONE  n = m = 1            k = 0         : corr(L0, R0)
+ *  XC_ONE_TO_MANY n = 1                k = t         : corr(L0, Rt)
+ *  XC_N_TO_MN     m % n == 0           k = t         : corr(L[t % n], Rt)   (file order)
+ *  XC_N_TO_M      any n, m             k = l*m + t   : corr(Ll, Rt)                      */
+typedef enum xc_family {
+    XC_ONE_TO_ONE = 0,
+    XC_ONE_TO_MANY = 1,
+    XC_N_TO_MN = 2,
+    XC_N_TO_M = 3
+} xc_family;
+
+/* Arithmetic used for float inputs.  Every method accumulates in FP32 with a
+ * two-level (per row-pair, then total) sum.
+ *  XC_PREC_FP32     exact FP32 products on the VALU (wave64 direct kernel)
+ *  XC_PREC_BF16X3   MFMA implicit GEMM, each operand split into bf16 hi + lo,
+ *                   hi*hi + hi*lo + lo*hi; outputs whose overlap is smaller than
+ *                   the border threshold are recomputed exactly in FP32
+ *  XC_PREC_BF16     MFMA, single bf16 product (fast, ~1e-3 relative error)
+ *  XC_PREC_FP16     MFMA, single fp16 product (inputs must fit fp16 range)
+ *  XC_PREC_FFT64    FFT method computed in FP64 (rocFFT D2Z/Z2D), results in FP32
+ *  XC_PREC_FFT32    FFT method in FP32 (rocFFT R2C/C2R); large error at low-overlap outputs
+ *  XC_PREC_AUTO     the library picks (cost model / autotuned table)           */
+typedef enum xc_precision {
+    XC_PREC_AUTO = 0,
+    XC_PREC_FP32 = 1,
+    XC_PREC_BF16X3 = 2,
+    XC_PREC_BF16 = 3,
+    XC_PREC_FP16 = 4,
+    XC_PREC_FFT64 = 5,
+    XC_PREC_FFT32 = 6
+} xc_precision;
+
+/* Optional tuning; zero-initialise and set what you need (0 = library default). */
+typedef struct xc_options {
+    uint32_t precision;   /* xc_precision */
+    uint32_t tile;        /* kernel-specific tile code from the tile catalog, 0 = default */
+    uint32_t split;       /* split of the reduction (overlap rows) across blocks, 0 = auto */
+    uint32_t border;      /* BF16X3: overlap-area threshold for exact FP32 recompute, 0 = default */
+    uint32_t reserved[4];
+} xc_options;
+
+/* ---- family entry points (the drop-in for the L5 wrappers' run_impl) ----------------
+ * Replace cpu_cross_corr_* (src/validate.hpp:216-298) and the run_* launchers of
+ * src/kernels.cuh:65-283 that serve each family.  n = number of left matrices,
+ * m = number of right matrices, out holds xc_num_results(family, n, m) results.   */
+xc_status xc_cross_corr_f32(xc_family family, const float* left, const float* right, float* out,
+                            uint32_t rows, uint32_t cols, uint32_t n, uint32_t m,
+                            const xc_options* opt, xc_stream_t stream);
+xc_status xc_cross_corr_f64(xc_family family, const double* left, const double* right, double* out,
+                            uint32_t rows, uint32_t cols, uint32_t n, uint32_t m,
+                            const xc_options* opt, xc_stream_t stream);
+
+/* Number of results of a family, or -1 if the counts are invalid for it
+ * (one_to_one_validate_input_size etc., src/one_to_one.hpp:20-22,
+ * src/one_to_many.hpp:15-17, src/n_to_mn.hpp:23-25, src/n_to_m.hpp:19-21). */
+int64_t xc_num_results(xc_family family, uint32_t n, uint32_t m);
+
+/* ---- uneven batch: pairs of different sizes in one persistent launch ----------------
+ * No counterpart in the reference (its n_to_mn wrapper launches one kernel per left
+ * matrix on round-robin streams, src/n_to_mn.hpp:553-571).  Pair i correlates
+ * left + left_off[i] with right + right_off[i] (both rows[i] x cols[i]) into
+ * out + out_off[i]; offsets are in elements and the descriptor arrays are HOST arrays. */
+typedef struct xc_pair {
+    uint64_t left_off, right_off, out_off;
+    uint32_t rows, cols;
+} xc_pair;
+
+xc_status xc_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* left, const float* right,
+                       float* out, const xc_options* opt, xc_stream_t stream);
+
+/* ---- FFT-method building blocks (src/kernels.cuh:7-63) ------------------------------ */
+/* run_scatter (src/kernels.cuh:53-63, src/scatter.cu:15-102): copy src matrices into
+ * the top-left (at dst_row, dst_col) of dst matrices and write zeros everywhere else
+ * in dst (the reference needs a prior cudaMemset of dst; this one does not). */
+xc_status xc_scatter_f32(const float* src, float* dst, uint32_t src_rows, uint32_t src_cols,
+                         uint32_t num_matrices, uint32_t dst_rows, uint32_t dst_cols,
+                         uint32_t dst_row, uint32_t dst_col, xc_stream_t stream);
+xc_status xc_scatter_f64(const double* src, double* dst, uint32_t src_rows, uint32_t src_cols,
+                         uint32_t num_matrices, uint32_t dst_rows, uint32_t dst_cols,
+                         uint32_t dst_row, uint32_t dst_col, xc_stream_t stream);
+/* run_hadamard_original (src/kernels.cuh:7-15, src/hadamard.cu:33-58,176-195):
+ * deformed[b][i] = ref[i] * conj(deformed[b][i]) for b < batch, i < num_complex.
+ * Complex values are interleaved (re, im) pairs. */
+xc_status xc_hadamard_f32(const float* ref, float* deformed, uint64_t num_complex, uint32_t batch,
+                          xc_stream_t stream);
+xc_status xc_hadamard_f64(const double* ref, double* deformed, uint64_t num_complex, uint32_t batch,
+                          xc_stream_t stream);
+/* run_hadamard_n_to_m_over_output (src/kernels.cuh:29-39, src/hadamard.cu:116-149):
+ * out[l*m + r][i] = left[l][i] * conj(right[r][i]). */
+xc_status xc_hadamard_n_to_m_f32(const float* left, const float* right, float* out, uint64_t num_complex,
+                                 uint32_t n, uint32_t m, xc_stream_t stream);
+xc_status xc_hadamard_n_to_m_f64(const double* left, const double* right, double* out, uint64_t num_complex,
+                                 uint32_t n, uint32_t m, xc_stream_t stream);
+/* normalize_fft_results (src/fft_helpers.hpp:193-261) on the GPU: res is `batch`
+ * unnormalised circular correlations of rows x cols (the padded size); norm gets
+ * batch results of (rows-1) x (cols-1). */
+xc_status xc_normalize_fft_f32(const float* res, float* norm, uint32_t rows, uint32_t cols, uint32_t batch,
+                               xc_stream_t stream);
+xc_status xc_normalize_fft_f64(const double* res, double* norm, uint32_t rows, uint32_t cols, uint32_t batch,
+                               xc_stream_t stream);
+/* Whole FFT method for a family (fft_reduced_transfer_* / fft_better_n_to_m,
+ * src/one_to_many.hpp:679-745, src/n_to_m.hpp:700-736).  If `normalize` is 0 the
+ * results are the unnormalised (2*rows) x (2*cols) circular correlations the
+ * reference stores without -n; otherwise (2*rows-1) x (2*cols-1) as every other
+ * method.  Plans and workspace are cached per (shape, batch, precision). */
+xc_status xc_fft_cross_corr_f32(xc_family family, const float* left, const float* right, float* out,
+                                uint32_t rows, uint32_t cols, uint32_t n, uint32_t m, int normalize,
+                                int fp64_transform, xc_stream_t stream);
+xc_status xc_fft_cross_corr_f64(xc_family family, const double* left, const double* right, double* out,
+                                uint32_t rows, uint32_t cols, uint32_t n, uint32_t m, int normalize,
+                                xc_stream_t stream);
+
+/* ---- reference run_* launchers with their tuning arguments --------------------------
+ * Same argument meaning as src/kernels.cuh; tuning arguments are validated with the
+ * reference's limits (e.g. warps_per_thread_block <= 32, src/naive_shuffle.cu:461-463)
+ * and mapped onto this library's kernels; matrix_size/search_size are {cols, rows}. */
+xc_status xc_run_ccn_shuffle_f32(const float* left, const float* right, float* out,
+                                 uint32_t cols, uint32_t rows, uint32_t search_cols, uint32_t search_rows,
+                                 uint32_t warps_per_thread_block, xc_stream_t stream);
+xc_status xc_run_ccn_shuffle_one_to_many_multirow_both_multimat_right_f32(
+    const float* left, const float* right, float* out, uint32_t cols, uint32_t rows,
+    uint32_t search_cols, uint32_t search_rows, uint32_t num_right_matrices,
+    uint32_t warps_per_thread_block, uint32_t shifts_per_thread_right_matrix,
+    uint32_t right_matrices_per_thread, uint32_t left_rows_per_iteration, xc_stream_t stream);
+
+/* ---- diagnostics (src/kernel_args.hpp:9-33) ----------------------------------------- */
+typedef struct xc_launch_desc {
+    char kernel[96];
+    uint32_t grid[3];
+    uint32_t block[3];
+    uint32_t lds_bytes;
+    uint32_t rows, cols, pairs, split;
+} xc_launch_desc;
+
+const char* xc_last_error(void);             /* thread-local; "" if none */
+const xc_launch_desc* xc_last_launch(void);  /* thread-local; last launch this thread enqueued */
+const char* xc_version(void);
+/* Describe the device the library runs on (name, CUs, clock); returns XC_ERR_HIP without a GPU. */
+xc_status xc_device_info(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XCORR_H */

@@ -175,3 +175,31 @@ def seeded(shape, dist, seed, dtype=np.float32):
     else:
         raise ValueError(dist)
     return a.astype(dtype)
+
+
+_PKG = None
+
+
+def xcorr():
+    """The product package (cross-corr_amd/__init__.py), imported by path."""
+    global _PKG
+    if _PKG is None:
+        import importlib.util
+        import sys
+        spec = importlib.util.spec_from_file_location("cross_corr_amd", REPO / "cross-corr_amd" / "__init__.py")
+        mod = importlib.util.module_from_spec(spec)
+        sys.modules["cross_corr_amd"] = mod
+        spec.loader.exec_module(mod)
+        _PKG = mod
+    return _PKG
+
+
+def header_symbols():
+    """Function names declared in include/*.h."""
+    import re
+    names = []
+    for h in sorted((REPO / "include").glob("*.h")):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"\b(xc_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
