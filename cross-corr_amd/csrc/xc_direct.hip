@@ -12,8 +12,9 @@
 //     sides, so every lane reads its sliding window R[j][c-dx..] without bounds checks
 //     (the reference's bound_checked_loads.cuh:8-53 become padding);
 //   * each thread keeps TY outputs; products of one row pair are summed in a fresh
-//     accumulator that is added to the total after the row (two-level summation keeps
-//     the FP32 error below 1e-6 of sum|a||b| where one running sum does not);
+//     accumulator that is added to the total after the row with a compensated (Kahan)
+//     add -- a single running FP32 sum, or a plain two-level one, exceeds 1e-6 of
+//     sum|a||b| at 256x256 for positive data;
 //   * the reduction over right rows can be split over `split` blocks that add their
 //     partial sums atomically into a zeroed output ("work distribution",
 //     src/naive_shuffle.cu:306-413, src/row_distribution.cuh:34-284, here a plain
@@ -131,16 +132,16 @@ __global__ __launch_bounds__(256) void direct_kernel(const T* __restrict__ L, co
     // LDS row: buf[P] = R[j][qb + P] (0 outside [0, W)), qb <= min over the block of c - dx.
     const int qb = -((dxb + kBX - 1 + 3) & ~3);
 
-    T tot[TY];
+    T tot[TY], comp[TY];
 #pragma unroll
-    for (int y = 0; y < TY; ++y) tot[y] = T(0);
+    for (int y = 0; y < TY; ++y) tot[y] = comp[y] = T(0);
 
     if (j_lo < j_hi) {
         auto fetch = [&](int j, int e) -> T {
             const int q = qb + e;
             return (q >= 0 && q < W) ? Rp[(size_t)j * W + q] : T(0);
         };
-        for (int e = threadIdx.x; e < len; e += 256) buf[e] = fetch(j_lo, e);
+        for (int e = threadIdx.x; e < len; e += 256) buf[(j_lo & 1) * len + e] = fetch(j_lo, e);
         __syncthreads();
 
         for (int j = j_lo; j < j_hi; ++j) {
@@ -160,8 +161,15 @@ __global__ __launch_bounds__(256) void direct_kernel(const T* __restrict__ L, co
             const int r0 = j + dy0;
             T acc[TY];
             row_pair<T, TY, CW>(acc, Lp, r0, H, W, cur, c_lo - dx - qb, c_lo, c_hi, c_full);
+            // Compensated (Kahan) add of the row sums: the error of summing the H row sums
+            // in FP32 dominates a plain two-level sum.
 #pragma unroll
-            for (int y = 0; y < TY; ++y) tot[y] += acc[y];
+            for (int y = 0; y < TY; ++y) {
+                const T yv = acc[y] - comp[y];
+                const T t = tot[y] + yv;
+                comp[y] = (t - tot[y]) - yv;
+                tot[y] = t;
+            }
 
             if (more) {
 #pragma unroll

@@ -68,10 +68,6 @@ void run_family_f32(const family_args& a, const float* L, const float* R, float*
     }
 }
 
-void launch_mfma_f32(const family_args&, const float*, const float*, float*, int, uint32_t, hipStream_t) {
-    fail(XC_ERR_UNSUPPORTED, "MFMA methods are not built yet");
-}
-
 void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, const float* R, float* O,
                    const xc_options& opt, hipStream_t s) {
     for (uint32_t i = 0; i < num_pairs; ++i) {
