@@ -15,7 +15,8 @@ void run_family_f32(const family_args& a, const float* L, const float* R, float*
 
 // Split-precision MFMA implicit GEMM (xc_mfma.hip); `kind` is an xc_precision value.
 void launch_mfma_f32(const family_args& a, const float* L, const float* R, float* O, int kind,
-                     uint32_t border, hipStream_t s);
+                     uint32_t border, uint32_t tile, hipStream_t s);
+int mfma_catalog_size();
 
 // Uneven batch (xc_batch.hip).
 void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, const float* R, float* O,

@@ -33,7 +33,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kKC = 128;              // columns c' per pass over the left rows
+constexpr int kMaxBorder = 512;       // largest accepted border threshold
 constexpr uint32_t kDefaultBorder = 512;  // overlap area below which bf16x3 outputs are recomputed
 
 template <int KIND>
@@ -75,16 +75,19 @@ __device__ __forceinline__ f32x16 mma(const uint4& a, const uint4& b, f32x16 c) 
                                                        c, 0, 0, 0);
 }
 
-// LDS geometry (bytes) of one block.
-template <int WX, int WY, int PLANES>
+// LDS geometry (bytes) of one block; KC columns c' per pass over the left rows.
+template <int WX, int WY, int PLANES, int KC>
 struct geom {
     static constexpr int BX = 64 * WX;                    // dx per block
     static constexpr int BY = 64 * WY;                    // dy per block
     static constexpr int NSLOT = BY + 1;                  // ring slots (right rows)
-    static constexpr int RS = kKC * 2 + 16;               // ring row stride: an odd multiple of 16 B
-    static constexpr int LENA = kKC + BX + 64;            // staged left-row elements per copy
+    static constexpr int RS = KC * 2 + 16;               // ring row stride: an odd multiple of 16 B
+    static constexpr int LENA = KC + BX + 64;            // staged left-row elements per copy
     static constexpr int CS = ((LENA * 2 + 255) / 256) * 256 + 64;  // copy stride, 64 mod 256
-    static constexpr int RING = NSLOT * PLANES * RS;
+    // Slot (one ring row, all planes) stride: an odd multiple of 16 B, so that the 32 lanes
+    // reading 32 consecutive slots hit distinct LDS banks.
+    static constexpr int SS = ((PLANES * RS / 16) % 2) ? PLANES * RS : PLANES * RS + 16;
+    static constexpr int RING = NSLOT * SS;
     static constexpr int LBUF = PLANES * 4 * CS;          // one left-row buffer (4 shifted copies)
     static constexpr int BYTES = RING + 2 * LBUF;
 };
@@ -94,14 +97,126 @@ __device__ __forceinline__ int ring_slot(int row, int nslot) {
     return s < 0 ? s + nslot : s;
 }
 
-template <int KIND, int WX, int WY>
-__global__ __launch_bounds__(256, 1) void mfma_kernel(const float* __restrict__ L, const float* __restrict__ R,
+// MFMAs of one left-row step over chunks [q0, q1) of 16 columns.  MASK bit 2u+s selects the
+// (dy sub-tile u, dx sub-tile s) accumulators this range feeds; fragments of the next chunk
+// are loaded while the current chunk's MFMAs run (the one-past-the-end load stays inside the
+// LDS ring / row buffers).
+template <int KIND, int P, int MASK, bool PIPE>
+__device__ __forceinline__ void chunk_range(f32x16 (&acc)[2][2], const char* const (&arow)[2],
+                                            const char* const (&bcol)[2], int q0, int q1, int rs, int cs) {
+    auto load = [&](int qq, uint4 (&a)[2][P], uint4 (&b)[2][P]) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int pl = 0; pl < P; ++pl)
+                if (MASK & (3 << (2 * u))) a[u][pl] = *reinterpret_cast<const uint4*>(arow[u] + pl * rs + qq * 2);
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int pl = 0; pl < P; ++pl)
+                if (MASK & (5 << s)) {
+                    const uint2* q = reinterpret_cast<const uint2*>(bcol[s] + pl * 4 * cs + qq * 2);
+                    const uint2 v0 = q[0], v1 = q[1];
+                    b[s][pl] = make_uint4(v0.x, v0.y, v1.x, v1.y);
+                }
+    };
+    uint4 a0[2][P], b0[2][P];
+    if (PIPE) load(q0, a0, b0);
+    for (int qq = q0; qq < q1; qq += 16) {
+        uint4 a1[2][P], b1[2][P];
+        if (PIPE) load(qq + 16, a1, b1);
+        else load(qq, a0, b0);
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                if (MASK & (1 << (2 * u + s))) acc[u][s] = mma<KIND>(a0[u][0], b0[s][0], acc[u][s]);
+        if constexpr (P == 2) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+                    if (MASK & (1 << (2 * u + s))) acc[u][s] = mma<KIND>(a0[u][0], b0[s][1], acc[u][s]);
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+                    if (MASK & (1 << (2 * u + s))) acc[u][s] = mma<KIND>(a0[u][1], b0[s][0], acc[u][s]);
+        }
+        if (PIPE) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int pl = 0; pl < P; ++pl) {
+                    a0[i][pl] = a1[i][pl];
+                    b0[i][pl] = b1[i][pl];
+                }
+        }
+    }
+}
+
+// Same with the mask known only at run time (uniform branches around each MFMA group).
+template <int KIND, int P>
+__device__ __forceinline__ void chunk_range_rt(int mask, f32x16 (&acc)[2][2], const char* const (&arow)[2],
+                                               const char* const (&bcol)[2], int q0, int q1, int rs, int cs) {
+    for (int qq = q0; qq < q1; qq += 16) {
+        uint4 a0[2][P], b0[2][P];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int pl = 0; pl < P; ++pl) a0[u][pl] = *reinterpret_cast<const uint4*>(arow[u] + pl * rs + qq * 2);
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int pl = 0; pl < P; ++pl) {
+                const uint2* q = reinterpret_cast<const uint2*>(bcol[s] + pl * 4 * cs + qq * 2);
+                const uint2 v0 = q[0], v1 = q[1];
+                b0[s][pl] = make_uint4(v0.x, v0.y, v1.x, v1.y);
+            }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                if (mask & (1 << (2 * u + s))) {
+                    acc[u][s] = mma<KIND>(a0[u][0], b0[s][0], acc[u][s]);
+                    if constexpr (P == 2) {
+                        acc[u][s] = mma<KIND>(a0[u][0], b0[s][1], acc[u][s]);
+                        acc[u][s] = mma<KIND>(a0[u][1], b0[s][0], acc[u][s]);
+                    }
+                }
+    }
+}
+
+template <int KIND, int P, bool PIPE>
+__device__ __forceinline__ void chunk_range_dispatch(int mask, f32x16 (&acc)[2][2], const char* const (&arow)[2],
+                                                     const char* const (&bcol)[2], int q0, int q1, int rs, int cs) {
+    if constexpr (!PIPE) {
+        if (mask == 15) chunk_range<KIND, P, 15, false>(acc, arow, bcol, q0, q1, rs, cs);
+        else chunk_range_rt<KIND, P>(mask, acc, arow, bcol, q0, q1, rs, cs);
+        return;
+    }
+    switch (mask) {
+#define XC_CASE(M) \
+    case M: chunk_range<KIND, P, M, PIPE>(acc, arow, bcol, q0, q1, rs, cs); break;
+        XC_CASE(1) XC_CASE(2) XC_CASE(3) XC_CASE(4) XC_CASE(5) XC_CASE(6) XC_CASE(7) XC_CASE(8)
+        XC_CASE(9) XC_CASE(10) XC_CASE(11) XC_CASE(12) XC_CASE(13) XC_CASE(14) XC_CASE(15)
+#undef XC_CASE
+    default: break;
+    }
+}
+
+// KC: columns per pass; FLUSH: left-row steps summed in `acc` before it is added to the
+// total; KAHAN: compensated add of those sums (else plain); OCC: blocks per CU the register
+// budget is sized for.
+template <int KIND, int WX, int WY, int KC, int FLUSH, bool KAHAN, int OCC>
+__global__ __launch_bounds__(256, OCC) void mfma_kernel(const float* __restrict__ L, const float* __restrict__ R,
                                                       float* __restrict__ O, int family, uint32_t rows,
                                                       uint32_t cols, uint32_t n, uint32_t m, uint32_t tiles_x,
                                                       uint32_t tiles_y) {
     using S = split_traits<KIND>;
     constexpr int P = S::planes;
-    using G = geom<WX, WY, P>;
+    using G = geom<WX, WY, P, KC>;
+    constexpr bool PIPE = OCC == 1;  // two waves per SIMD hide LDS latency without it
     extern __shared__ __attribute__((aligned(16))) char smem[];
     char* ring = smem;
     char* lbuf = smem + G::RING;
@@ -125,7 +240,9 @@ __global__ __launch_bounds__(256, 1) void mfma_kernel(const float* __restrict__ 
     const int lane = threadIdx.x & 63;
     const int li32 = lane & 31, g = lane >> 5;
     const int wx = wave % WX, wy = wave / WX;
-    const int dxw = dxb + 64 * wx;
+    // The wave's two 32-wide dx sub-tiles are 32*WX apart, so that every wave of the block
+    // gets one sub-tile from each half of the block's dx range (balanced work per step).
+    const int dxs[2] = {dxb + 32 * wx, dxb + 32 * wx + 32 * WX};
     const int dyw = dyb + 64 * wy;
 
     // Left rows meeting the block's dy range: r in [max(0,dy), min(H, H+dy)).
@@ -135,169 +252,187 @@ __global__ __launch_bounds__(256, 1) void mfma_kernel(const float* __restrict__ 
     // Columns c' meeting the block's / the wave's dx range: c' in [max(0,-dx), min(W, W-dx)).
     const int dx_hi_b = min(dxb + G::BX - 1, W - 1);
     const int cb_lo = max(0, -dx_hi_b), cb_hi = min(W, W - dxb);
-    const int dx_hi_w = min(dxw + 63, W - 1);
-    const int cw_lo = max(0, -dx_hi_w), cw_hi = min(W, W - dxw);
+    int cw_lo = W, cw_hi = 0;
+#pragma unroll
+    for (int sidx = 0; sidx < 2; ++sidx) {
+        const int lo = dxs[sidx], hi = min(lo + 31, W - 1);
+        if (lo <= hi) {
+            cw_lo = min(cw_lo, max(0, -hi));
+            cw_hi = max(cw_hi, min(W, W - lo));
+        }
+    }
 
-    f32x16 tot[2][2], comp[2][2];
+    f32x16 tot[2][2], comp[2][2], acc[2][2];
 #pragma unroll
     for (int u = 0; u < 2; ++u)
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll
-            for (int e = 0; e < 16; ++e) tot[u][s][e] = comp[u][s][e] = 0.f;
+            for (int e = 0; e < 16; ++e) tot[u][s][e] = comp[u][s][e] = acc[u][s][e] = 0.f;
 
-    // Stage right row `row` (columns [c0, c0+kKC)) into its ring slot.
-    auto stage_right = [&](int row, int c0, int e, uint16_t* hi, uint16_t* lo) {
-        const int c = c0 + e;
-        float v = 0.f;
-        if ((unsigned)row < (unsigned)H && c < W) v = Rp[(size_t)row * W + c];
-        S::split(v, *hi, *lo);
-    };
-    auto ring_put = [&](int row, int e, uint16_t hi, uint16_t lo) {
-        char* base = ring + ring_slot(row, G::NSLOT) * P * G::RS + e * 2;
-        *reinterpret_cast<uint16_t*>(base) = hi;
-        if (P == 2) *reinterpret_cast<uint16_t*>(base + G::RS) = lo;
-    };
-    // Left row r, element e of the staged window starting at column cb: written to the four
-    // shifted copies (copy t holds column cb + p + t at position p).
-    auto lbuf_put = [&](int buf, int e, uint16_t hi, uint16_t lo) {
-        char* base = lbuf + buf * G::LBUF;
+    // Staging works on quads of 4 consecutive elements, written as one 8-byte LDS store per
+    // plane.  Left row: item (copy t, quad j) holds columns cbase + 4j + t .. +3 at positions
+    // 4j .. 4j+3 of copy t.  Right row: quad j holds columns c0 + 4j .. +3 of its ring slot.
+    static_assert(G::LENA % 4 == 0 && KC % 4 == 0, "quad staging");
+    constexpr int LQ = G::LENA / 4;       // quads per left copy
+    constexpr int LITEMS = 4 * LQ;        // left items per row
+    constexpr int LPT = (LITEMS + 255) / 256;
+    constexpr int RQ = KC / 4;           // quads per right row
+    auto load_left = [&](int r, int cbase, int item) -> float4 {
+        const int t = item / LQ, j = item - (item / LQ) * LQ;
+        const int c = cbase + 4 * j + t;
+        float v[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int p = e - t;
-            if (p >= 0 && p < G::LENA) {
-                *reinterpret_cast<uint16_t*>(base + t * G::CS + p * 2) = hi;
-                if (P == 2) *reinterpret_cast<uint16_t*>(base + 4 * G::CS + t * G::CS + p * 2) = lo;
-            }
-        }
+        for (int k = 0; k < 4; ++k)
+            v[k] = ((unsigned)r < (unsigned)H && (unsigned)(c + k) < (unsigned)W) ? Lp[(size_t)r * W + c + k] : 0.f;
+        return make_float4(v[0], v[1], v[2], v[3]);
     };
-    auto left_val = [&](int r, int c) -> float {
-        return ((unsigned)r < (unsigned)H && (unsigned)c < (unsigned)W) ? Lp[(size_t)r * W + c] : 0.f;
+    auto load_right = [&](int row, int c0, int j) -> float4 {
+        const int c = c0 + 4 * j;
+        float v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            v[k] = ((unsigned)row < (unsigned)H && c + k < W) ? Rp[(size_t)row * W + c + k] : 0.f;
+        return make_float4(v[0], v[1], v[2], v[3]);
+    };
+    auto split4 = [&](float4 v, uint2& hi, uint2& lo) {
+        uint16_t h[4], l[4];
+        S::split(v.x, h[0], l[0]);
+        S::split(v.y, h[1], l[1]);
+        S::split(v.z, h[2], l[2]);
+        S::split(v.w, h[3], l[3]);
+        hi = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+        lo = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
+    };
+    auto put_left = [&](int buf, int item, float4 v) {
+        const int t = item / LQ, j = item - (item / LQ) * LQ;
+        uint2 hi, lo;
+        split4(v, hi, lo);
+        char* base = lbuf + buf * G::LBUF + t * G::CS + j * 8;
+        *reinterpret_cast<uint2*>(base) = hi;
+        if (P == 2) *reinterpret_cast<uint2*>(base + 4 * G::CS) = lo;
+    };
+    auto put_right = [&](int row, int j, float4 v) {
+        uint2 hi, lo;
+        split4(v, hi, lo);
+        char* base = ring + ring_slot(row, G::NSLOT) * G::SS + j * 8;
+        *reinterpret_cast<uint2*>(base) = hi;
+        if (P == 2) *reinterpret_cast<uint2*>(base + G::RS) = lo;
     };
 
-    constexpr int LE = G::LENA + 3;  // elements loaded per left row
-    constexpr int LPT = (LE + 255) / 256;
-    constexpr int RPT = kKC / 256 > 0 ? kKC / 256 : 1;
-
-    for (int c0 = (cb_lo / kKC) * kKC; c0 < cb_hi; c0 += kKC) {
+    for (int c0 = (cb_lo / KC) * KC; c0 < cb_hi; c0 += KC) {
         const int cbase = c0 + dxb;  // left column at staged position 0
         // Prefill: right rows r_lo - dy for dy in the block (the ring), left row r_lo.
-        for (int idx = threadIdx.x; idx < G::BY * kKC; idx += 256) {
-            const int ro = idx / kKC, e = idx % kKC;
+        for (int idx = threadIdx.x; idx < G::BY * RQ; idx += 256) {
+            const int ro = idx / RQ, j = idx - (idx / RQ) * RQ;
             const int row = r_lo - dyb - ro;
-            uint16_t hi, lo;
-            stage_right(row, c0, e, &hi, &lo);
-            ring_put(row, e, hi, lo);
+            put_right(row, j, load_right(row, c0, j));
         }
-        for (int e = threadIdx.x; e < LE; e += 256) {
-            uint16_t hi, lo;
-            S::split(left_val(r_lo, cbase + e), hi, lo);
-            lbuf_put(0, e, hi, lo);
-        }
+        for (int item = threadIdx.x; item < LITEMS; item += 256) put_left(0, item, load_left(r_lo, cbase, item));
         __syncthreads();
 
-        // Wave's K chunks of this pass.
-        const int q_lo = max(cw_lo, c0), q_hi = min(cw_hi, c0 + kKC);
-        const int qk_lo = ((q_lo - c0) & ~15), qk_hi = q_hi - c0;
+        // Wave's K chunks of this pass, and those of each dx sub-tile (offsets from c0,
+        // rounded out to chunks of 16; empty when the sub-tile is outside the output).
+        const int q_lo = max(cw_lo, c0), q_hi = min(cw_hi, c0 + KC);
+        int ks_lo[2], ks_hi[2];
+#pragma unroll
+        for (int sidx = 0; sidx < 2; ++sidx) {
+            const int lo = dxs[sidx], hi = min(lo + 31, W - 1);
+            const int a0 = max(max(0, -hi), c0) - c0, a1 = min(min(W, W - lo), c0 + KC) - c0;
+            if (lo <= hi && a0 < a1) {
+                ks_lo[sidx] = a0 & ~15;
+                ks_hi[sidx] = (a1 + 15) & ~15;
+            } else {
+                ks_lo[sidx] = ks_hi[sidx] = 0;
+            }
+        }
 
         for (int r = r_lo; r < r_hi; ++r) {
             const int cur = (r - r_lo) & 1;
             const bool more = r + 1 < r_hi;
             // Prefetch next step's data into registers.
-            float pl[LPT];
-            float pr[RPT];
+            float4 pl[LPT];
+            float4 pr = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
             for (int i = 0; i < LPT; ++i) {
-                const int e = threadIdx.x + i * 256;
-                pl[i] = (more && e < LE) ? left_val(r + 1, cbase + e) : 0.f;
+                const int item = threadIdx.x + i * 256;
+                pl[i] = (more && item < LITEMS) ? load_left(r + 1, cbase, item) : make_float4(0.f, 0.f, 0.f, 0.f);
             }
+            if (more && threadIdx.x < RQ) pr = load_right(r + 1 - dyb, c0, threadIdx.x);
+
+            // Sub-tiles (32 dy x 32 dx) this step touches: left row r meets dy sub-tile u,
+            // chunk [c0+qq, c0+qq+16) meets the columns of dx sub-tile s.
+            bool act_u[2];
 #pragma unroll
-            for (int i = 0; i < RPT; ++i) {
-                const int e = threadIdx.x + i * 256;
-                const int row = r + 1 - dyb;
-                pr[i] = (more && e < kKC && (unsigned)row < (unsigned)H && c0 + e < W) ? Rp[(size_t)row * W + c0 + e]
-                                                                                        : 0.f;
+            for (int u = 0; u < 2; ++u) {
+                const int lo = dyw + 32 * u, hi = min(lo + 31, H - 1);
+                act_u[u] = lo <= hi && r >= max(0, lo) && r < min(H, H + hi);
             }
-
-            f32x16 acc[2][2];
-#pragma unroll
-            for (int u = 0; u < 2; ++u)
-#pragma unroll
-                for (int s = 0; s < 2; ++s)
-#pragma unroll
-                    for (int e = 0; e < 16; ++e) acc[u][s][e] = 0.f;
-
-            if (q_lo < q_hi) {
+            if (q_lo < q_hi && (act_u[0] || act_u[1])) {
                 const char* lb = lbuf + cur * G::LBUF;
                 // Ring rows of this lane for the two dy sub-tiles: row = r - (dyw + 32u + li32).
                 const char* arow[2];
 #pragma unroll
                 for (int u = 0; u < 2; ++u)
-                    arow[u] = ring + ring_slot(r - (dyw + 32 * u + li32), G::NSLOT) * P * G::RS + g * 16;
-                for (int qq = qk_lo; qq < qk_hi; qq += 16) {
-                    uint4 a[2][P], b[2][P];
-#pragma unroll
-                    for (int u = 0; u < 2; ++u)
-#pragma unroll
-                        for (int pl_ = 0; pl_ < P; ++pl_)
-                            a[u][pl_] = *reinterpret_cast<const uint4*>(arow[u] + pl_ * G::RS + qq * 2);
-#pragma unroll
-                    for (int s = 0; s < 2; ++s) {
-                        const int p = qq + 8 * g + (dxw - dxb) + 32 * s + li32;
-                        const int t = p & 3;
-#pragma unroll
-                        for (int pl_ = 0; pl_ < P; ++pl_) {
-                            const uint2* q =
-                                reinterpret_cast<const uint2*>(lb + pl_ * 4 * G::CS + t * G::CS + (p - t) * 2);
-                            const uint2 v0 = q[0], v1 = q[1];
-                            b[s][pl_] = make_uint4(v0.x, v0.y, v1.x, v1.y);
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < 2; ++u)
-#pragma unroll
-                        for (int s = 0; s < 2; ++s) acc[u][s] = mma<KIND>(a[u][0], b[s][0], acc[u][s]);
-                    if constexpr (P == 2) {
-#pragma unroll
-                        for (int u = 0; u < 2; ++u)
-#pragma unroll
-                            for (int s = 0; s < 2; ++s) {
-                                acc[u][s] = mma<KIND>(a[u][0], b[s][1], acc[u][s]);
-                                acc[u][s] = mma<KIND>(a[u][1], b[s][0], acc[u][s]);
-                            }
-                    }
-                }
-            }
-            // Compensated (Kahan) add of this step's sums into the totals.
-#pragma unroll
-            for (int u = 0; u < 2; ++u)
+                    arow[u] = ring + ring_slot(r - (dyw + 32 * u + li32), G::NSLOT) * G::SS + g * 16;
+                const char* bcol[2];
 #pragma unroll
                 for (int s = 0; s < 2; ++s) {
-                    const f32x16 yv = acc[u][s] - comp[u][s];
-                    const f32x16 t = tot[u][s] + yv;
-                    comp[u][s] = (t - tot[u][s]) - yv;
-                    tot[u][s] = t;
+                    const int p = 8 * g + (dxs[s] - dxb) + li32;  // + qq (a multiple of 16)
+                    const int t = p & 3;
+                    bcol[s] = lb + t * G::CS + (p - t) * 2;
                 }
+                // Split the chunk range where the set of active dx sub-tiles changes.  Sub-tile 1
+                // lies at larger dx than sub-tile 0, so its column range starts and ends no later.
+                const int umask = (act_u[0] ? 3 : 0) | (act_u[1] ? 12 : 0);
+                const int k0lo = ks_lo[0], k0hi = ks_hi[0], k1lo = ks_lo[1], k1hi = ks_hi[1];
+                const bool e0 = k0lo >= k0hi;  // sub-tile 0 has no columns in this pass
+                for (int seg = 0; seg < 3; ++seg) {
+                    int q0, q1, mask;
+                    if (seg == 0) {  // sub-tile 1 only
+                        q0 = k1lo;
+                        q1 = e0 ? k1hi : min(k1hi, k0lo);
+                        mask = umask & 10;
+                    } else if (seg == 1) {  // both
+                        q0 = e0 ? 0 : max(k0lo, k1lo);
+                        q1 = e0 ? 0 : min(k0hi, k1hi);
+                        mask = umask;
+                    } else {  // sub-tile 0 only
+                        q0 = e0 ? 0 : max(k0lo, k1hi);
+                        q1 = e0 ? 0 : k0hi;
+                        mask = umask & 5;
+                    }
+                    if (q0 < q1 && mask)
+                        chunk_range_dispatch<KIND, P, PIPE>(mask, acc, arow, bcol, q0, q1, G::RS, G::CS);
+                }
+            }
+            // Every FLUSH steps (and at the end of a pass) the step sums are added to the
+            // totals with a compensated (Kahan) add.
+            if (((r - r_lo + 1) % FLUSH) == 0 || r + 1 == r_hi) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        if constexpr (KAHAN) {
+                            const f32x16 yv = acc[u][s] - comp[u][s];
+                            const f32x16 t = tot[u][s] + yv;
+                            comp[u][s] = (t - tot[u][s]) - yv;
+                            tot[u][s] = t;
+                        } else {
+                            tot[u][s] += acc[u][s];
+                        }
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) acc[u][s][e] = 0.f;
+                    }
+            }
 
             if (more) {
 #pragma unroll
                 for (int i = 0; i < LPT; ++i) {
-                    const int e = threadIdx.x + i * 256;
-                    if (e < LE) {
-                        uint16_t hi, lo;
-                        S::split(pl[i], hi, lo);
-                        lbuf_put(cur ^ 1, e, hi, lo);
-                    }
+                    const int item = threadIdx.x + i * 256;
+                    if (item < LITEMS) put_left(cur ^ 1, item, pl[i]);
                 }
-#pragma unroll
-                for (int i = 0; i < RPT; ++i) {
-                    const int e = threadIdx.x + i * 256;
-                    if (e < kKC) {
-                        uint16_t hi, lo;
-                        S::split(pr[i], hi, lo);
-                        ring_put(r + 1 - dyb, e, hi, lo);
-                    }
-                }
+                if (threadIdx.x < RQ) put_right(r + 1 - dyb, threadIdx.x, pr);
             }
             __syncthreads();
         }
@@ -307,7 +442,7 @@ __global__ __launch_bounds__(256, 1) void mfma_kernel(const float* __restrict__ 
     // 8*(a/4) + 4*(lane/32) + a%4 (dy).
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-        const int dx = dxw + 32 * s + li32;
+        const int dx = dxs[s] + li32;
         if (dx > W - 1) continue;
 #pragma unroll
         for (int u = 0; u < 2; ++u)
@@ -319,45 +454,70 @@ __global__ __launch_bounds__(256, 1) void mfma_kernel(const float* __restrict__ 
     }
 }
 
-// Exact FP32 recomputation of the outputs whose overlap area (H-|dy|)*(W-|dx|) is below
-// `thresh`.  One block per (pair, dy); threads walk that row's frame outputs.
-__global__ __launch_bounds__(64) void frame_kernel(const float* __restrict__ L, const float* __restrict__ R,
-                                                   float* __restrict__ O, int family, uint32_t rows, uint32_t cols,
-                                                   uint32_t n, uint32_t m, uint32_t thresh) {
+// Frame outputs of a (rows, cols, threshold): the overlap sizes (h, w) with h*w < threshold,
+// enumerated h-major; start[h-1] is the index of (h, 1) and start[nh] the total count.
+struct frame_table {
+    uint32_t nh;
+    uint32_t start[kMaxBorder + 1];
+};
+
+// Exact FP32 recomputation of the outputs whose overlap area (H-|dy|)*(W-|dx|) is below the
+// threshold: one wave per (pair, overlap size (h, w), quadrant).  The wave's lanes cover the
+// h*w products (whole rows when w < 64), each lane sums its share in FP32 and the 64 lane
+// sums are added by a butterfly reduction.
+__global__ __launch_bounds__(256) void frame_kernel(const float* __restrict__ L, const float* __restrict__ R,
+                                                    float* __restrict__ O, int family, uint32_t rows,
+                                                    uint32_t cols, uint32_t n, uint32_t m, uint32_t pairs,
+                                                    frame_table tab) {
     const int H = (int)rows, W = (int)cols;
     const int SY = 2 * H - 1, SX = 2 * W - 1;
-    const uint32_t k = blockIdx.x / SY;
-    const int dy = (int)(blockIdx.x % SY) - (H - 1);
-    uint32_t li, ri;
-    pair_of(family, n, m, k, li, ri);
-    const float* __restrict__ Lp = L + (size_t)li * H * W;
-    const float* __restrict__ Rp = R + (size_t)ri * H * W;
-    float* __restrict__ Op = O + (size_t)k * SX * SY;
-    const int h = H - abs(dy);
-    const int wmax = min(W, (int)((thresh + h - 1) / h) - 1);  // overlap widths w with h*w < thresh
-    if (wmax <= 0) return;
-    const int count = wmax >= W ? SX : 2 * wmax;
-    const int r0 = max(0, dy), r1 = min(H, H + dy);
-    for (int f = threadIdx.x; f < count; f += 64) {
-        int dx;
-        if (wmax >= W) dx = f - (W - 1);
-        else dx = f < wmax ? -(W - 1) + f : (W - 1) - (f - wmax);
-        const int c_lo = max(0, dx), c_hi = min(W, W + dx);
-        float tot = 0.f;
-        for (int r = r0; r < r1; ++r) {
-            const float* lr = Lp + (size_t)r * W;
-            const float* rr = Rp + (size_t)(r - dy) * W - dx;
-            float acc = 0.f;
-            for (int c = c_lo; c < c_hi; ++c) acc = fmaf(lr[c], rr[c], acc);
-            tot += acc;
-        }
-        Op[(size_t)(dy + H - 1) * SX + (dx + W - 1)] = tot;
+    const int lane = threadIdx.x & 63;
+    const uint32_t o = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t hw = o >> 2, q = o & 3;
+    if (hw >= tab.start[tab.nh]) return;
+    // h: the largest h with start[h-1] <= hw
+    int lo = 0, hi = (int)tab.nh - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab.start[mid] <= hw) lo = mid;
+        else hi = mid - 1;
+    }
+    const int h = lo + 1, w = (int)(hw - tab.start[lo]) + 1;
+    const int dy = (q & 1) ? -(H - h) : (H - h);
+    const int dx = (q & 2) ? -(W - w) : (W - w);
+    if (((q & 1) && dy == 0) || ((q & 2) && dx == 0)) return;  // the same output as another quadrant
+    const int r0 = max(0, dy), c0 = max(0, dx);
+    // lane -> (first row, column) and row step
+    int rs, cl, rstep, cstep;
+    if (w >= 64) {
+        rs = 0;
+        cl = lane;
+        rstep = 1;
+        cstep = 64;
+    } else {
+        rstep = 64 / w;
+        rs = lane / w;
+        cl = lane - rs * w;
+        cstep = w;
+        if (rs >= rstep) rs = h;  // idle lane
+    }
+    for (uint32_t k = blockIdx.y; k < pairs; k += gridDim.y) {
+        uint32_t li, ri;
+        pair_of(family, n, m, k, li, ri);
+        const float* __restrict__ lp = L + (size_t)li * H * W + (size_t)r0 * W + c0;
+        const float* __restrict__ rp = R + (size_t)ri * H * W + (size_t)(r0 - dy) * W + (c0 - dx);
+        float acc = 0.f;
+        for (int r = rs; r < h; r += rstep)
+            for (int c = cl; c < w; c += cstep) acc = fmaf(lp[(size_t)r * W + c], rp[(size_t)r * W + c], acc);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) O[(size_t)k * SX * SY + (size_t)(dy + H - 1) * SX + (dx + W - 1)] = acc;
     }
 }
 
-template <int KIND, int WX, int WY>
+template <int KIND, int WX, int WY, int KC, int FLUSH, bool KAHAN, int OCC>
 void launch(const family_args& a, const float* L, const float* R, float* O, hipStream_t s) {
-    using G = geom<WX, WY, split_traits<KIND>::planes>;
+    using G = geom<WX, WY, split_traits<KIND>::planes, KC>;
     const uint32_t SX = 2 * a.cols - 1, SY = 2 * a.rows - 1;
     const uint32_t tiles_x = (SX + G::BX - 1) / G::BX;
     const uint32_t tiles_y = (SY + G::BY - 1) / G::BY;
@@ -365,7 +525,7 @@ void launch(const family_args& a, const float* L, const float* R, float* O, hipS
     if (nblocks > 0x7fffffffULL) fail(XC_ERR_INVALID_ARGUMENT, "problem too large for one launch");
     static bool attr_set = false;
     if (!attr_set) {
-        XC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&mfma_kernel<KIND, WX, WY>),
+        XC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&mfma_kernel<KIND, WX, WY, KC, FLUSH, KAHAN, OCC>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, G::BYTES));
         attr_set = true;
     }
@@ -373,7 +533,7 @@ void launch(const family_args& a, const float* L, const float* R, float* O, hipS
     record_launch(KIND == XC_PREC_BF16X3 ? "mfma_kernel<bf16x3>" : (KIND == XC_PREC_BF16 ? "mfma_kernel<bf16>"
                                                                                           : "mfma_kernel<fp16>"),
                   grid, block, G::BYTES, a.rows, a.cols, a.pairs, 1);
-    hipLaunchKernelGGL((mfma_kernel<KIND, WX, WY>), grid, block, G::BYTES, s, L, R, O, a.family, a.rows, a.cols, a.n,
+    hipLaunchKernelGGL((mfma_kernel<KIND, WX, WY, KC, FLUSH, KAHAN, OCC>), grid, block, G::BYTES, s, L, R, O, a.family, a.rows, a.cols, a.n,
                        a.m, tiles_x, tiles_y);
     XC_HIP(hipGetLastError());
 }
@@ -381,26 +541,59 @@ void launch(const family_args& a, const float* L, const float* R, float* O, hipS
 }  // namespace
 
 void launch_frame_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t thresh, hipStream_t s) {
-    const uint64_t nblocks = (uint64_t)a.pairs * (2 * a.rows - 1);
-    if (nblocks == 0 || thresh == 0) return;
-    if (nblocks > 0x7fffffffULL) fail(XC_ERR_INVALID_ARGUMENT, "problem too large for one launch");
-    hipLaunchKernelGGL(frame_kernel, dim3((uint32_t)nblocks), dim3(64), 0, s, L, R, O, a.family, a.rows, a.cols,
-                       a.n, a.m, thresh);
+    if (a.pairs == 0 || thresh <= 1) return;
+    if (thresh > (uint32_t)kMaxBorder) fail(XC_ERR_INVALID_ARGUMENT, "border threshold above " + std::to_string(kMaxBorder));
+    frame_table tab{};
+    uint32_t total = 0, nh = 0;
+    for (uint32_t h = 1; h <= a.rows && h < thresh; ++h) {
+        const uint32_t cnt = std::min<uint32_t>(a.cols, (thresh + h - 1) / h - 1);
+        if (cnt == 0) break;
+        tab.start[nh++] = total;
+        total += cnt;
+    }
+    if (nh == 0) return;
+    tab.nh = nh;
+    tab.start[nh] = total;
+    const uint32_t blocks = (uint32_t)(((uint64_t)total * 4 + 3) / 4);  // one wave per (h, w, quadrant)
+    dim3 grid(blocks, std::min<uint32_t>(a.pairs, 65535));
+    hipLaunchKernelGGL(frame_kernel, grid, dim3(256), 0, s, L, R, O, a.family, a.rows, a.cols, a.n, a.m, a.pairs,
+                       tab);
     XC_HIP(hipGetLastError());
 }
 
+// Tile catalog of the split-precision kernel (the runtime replacement of the reference's
+// compile-time template limits, CMakeLists.txt:16-45): xc_options.tile selects an entry,
+// 0 is the default.
+int mfma_catalog_size() { return 6; }
+
+template <int KIND>
+void launch_tile(const family_args& a, const float* L, const float* R, float* O, uint32_t tile, hipStream_t s) {
+    switch (tile) {
+    case 0:
+    case 4: launch<KIND, 4, 1, 128, 8, false, 2>(a, L, R, O, s); break;
+    case 1: launch<KIND, 4, 1, 128, 4, false, 2>(a, L, R, O, s); break;
+    case 2: launch<KIND, 4, 1, 256, 2, true, 1>(a, L, R, O, s); break;
+    case 3: launch<KIND, 4, 1, 128, 2, true, 1>(a, L, R, O, s); break;
+    case 5: launch<KIND, 4, 1, 64, 8, false, 2>(a, L, R, O, s); break;
+    case 6: launch<KIND, 2, 2, 64, 4, false, 2>(a, L, R, O, s); break;
+    default: fail(XC_ERR_INVALID_ARGUMENT, "tile code " + std::to_string(tile) + " is not in the MFMA tile catalog");
+    }
+}
+
 void launch_mfma_f32(const family_args& a, const float* L, const float* R, float* O, int kind, uint32_t border,
-                     hipStream_t s) {
+                     uint32_t tile, hipStream_t s) {
+    if (tile > (uint32_t)mfma_catalog_size())
+        fail(XC_ERR_INVALID_ARGUMENT, "tile code " + std::to_string(tile) + " is not in the MFMA tile catalog");
     switch (kind) {
     case XC_PREC_BF16X3:
-        launch<XC_PREC_BF16X3, 4, 1>(a, L, R, O, s);
+        launch_tile<XC_PREC_BF16X3>(a, L, R, O, tile, s);
         launch_frame_f32(a, L, R, O, border ? border : kDefaultBorder, s);
         break;
     case XC_PREC_BF16:
-        launch<XC_PREC_BF16, 4, 1>(a, L, R, O, s);
+        launch_tile<XC_PREC_BF16>(a, L, R, O, tile, s);
         break;
     case XC_PREC_FP16:
-        launch<XC_PREC_FP16, 4, 1>(a, L, R, O, s);
+        launch_tile<XC_PREC_FP16>(a, L, R, O, tile, s);
         break;
     default:
         fail(XC_ERR_INVALID_ARGUMENT, "not an MFMA precision");
