@@ -57,6 +57,10 @@ SIGNATURES = {
     "xc_cross_corr_f64": (_st, [_i32, _vp, _vp, _vp, _u32, _u32, _u32, _u32, ctypes.POINTER(Options), _vp]),
     "xc_num_results": (ctypes.c_int64, [_i32, _u32, _u32]),
     "xc_batch_f32": (_st, [ctypes.POINTER(Pair), _u32, _vp, _vp, _vp, ctypes.POINTER(Options), _vp]),
+    "xc_batch_plan_create_f32": (_st, [ctypes.POINTER(Pair), _u32, ctypes.POINTER(Options),
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    "xc_batch_plan_exec_f32": (_st, [_vp, _vp, _vp, _vp, _i32, _vp]),
+    "xc_batch_plan_destroy": (_st, [_vp]),
     "xc_scatter_f32": (_st, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _vp]),
     "xc_scatter_f64": (_st, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _vp]),
     "xc_hadamard_f32": (_st, [_vp, _vp, _u64, _u32, _vp]),
@@ -181,3 +185,29 @@ def batch(pairs, left_flat, right_flat, out_flat, precision="auto", stream=None)
     check(lib().xc_batch_f32(arr, len(pairs), _ptr(left_flat), _ptr(right_flat), _ptr(out_flat),
                              ctypes.byref(opt), _stream(stream)))
     return out_flat
+
+
+class BatchPlan:
+    """Reusable uneven-batch plan (persistent kernel, replayed as a HIP graph)."""
+
+    def __init__(self, pairs, precision="bf16x3", border=0):
+        arr = (Pair * len(pairs))(*[Pair(*p) for p in pairs])
+        opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, 0, 0, border)
+        self._h = ctypes.c_void_p()
+        check(lib().xc_batch_plan_create_f32(arr, len(pairs), ctypes.byref(opt), ctypes.byref(self._h)))
+
+    def run(self, left_flat, right_flat, out_flat, use_graph=True, stream=None):
+        check(lib().xc_batch_plan_exec_f32(self._h, _ptr(left_flat), _ptr(right_flat), _ptr(out_flat),
+                                           int(use_graph), _stream(stream)))
+        return out_flat
+
+    def close(self):
+        if self._h:
+            check(lib().xc_batch_plan_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -126,6 +126,26 @@ xc_status xc_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* le
     });
 }
 
+xc_status xc_batch_plan_create_f32(const xc_pair* pairs, uint32_t num_pairs, const xc_options* opt,
+                                   xc_batch_plan** plan) {
+    return guarded([&] {
+        if (!plan) xc::fail(XC_ERR_INVALID_ARGUMENT, "null plan output pointer");
+        *plan = nullptr;
+        xc_options o{};
+        if (opt) o = *opt;
+        *plan = xc::batch_plan_create(pairs, num_pairs, o);
+    });
+}
+
+xc_status xc_batch_plan_exec_f32(xc_batch_plan* plan, const float* left, const float* right, float* out, int use_graph,
+                                 xc_stream_t stream) {
+    return guarded([&] { xc::batch_plan_exec(plan, left, right, out, (hipStream_t)stream, use_graph != 0); });
+}
+
+xc_status xc_batch_plan_destroy(xc_batch_plan* plan) {
+    return guarded([&] { delete plan; });
+}
+
 xc_status xc_scatter_f32(const float* src, float* dst, uint32_t src_rows, uint32_t src_cols, uint32_t num_matrices,
                          uint32_t dst_rows, uint32_t dst_cols, uint32_t dst_row, uint32_t dst_col,
                          xc_stream_t stream) {

@@ -1,0 +1,230 @@
+// Uneven batch: pairs of different sizes in one persistent launch, captured once into a HIP
+// graph and replayed.  No counterpart in the reference, whose n_to_mn wrapper launches one
+// kernel per left matrix on round-robin streams (src/n_to_mn.hpp:553-571).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "xc_methods.hpp"
+
+struct xc_batch_plan {
+    int device = 0;
+    int kind = XC_PREC_BF16X3;
+    uint32_t thresh = 0;
+    uint32_t n_items = 0;
+    uint32_t grid = 0, lds = 0;
+    xc::batch_item* d_items = nullptr;
+    xc::batch_pair_desc* d_pairs = nullptr;
+    uint32_t* d_tables = nullptr;
+    uint32_t* d_counters = nullptr;
+    hipStream_t capture_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t node = nullptr;
+    const float* L = nullptr;
+    const float* R = nullptr;
+    float* O = nullptr;
+    std::vector<xc_pair> pairs;
+
+    ~xc_batch_plan() {
+        // work launched from this plan may still be queued: let the device drain first
+        (void)hipDeviceSynchronize();
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (capture_stream) (void)hipStreamDestroy(capture_stream);
+        for (void* p : {(void*)d_items, (void*)d_pairs, (void*)d_tables, (void*)d_counters})
+            if (p) (void)hipFree(p);
+    }
+};
+
+namespace xc {
+
+namespace {
+
+void check_pairs(const xc_pair* pairs, uint32_t n) {
+    if (n && !pairs) fail(XC_ERR_INVALID_ARGUMENT, "null pair array");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (pairs[i].rows == 0 || pairs[i].cols == 0)
+            fail(XC_ERR_INVALID_ARGUMENT, "pair " + std::to_string(i) + " has an empty matrix");
+        if (pairs[i].rows > 32768 || pairs[i].cols > 32768)
+            fail(XC_ERR_INVALID_ARGUMENT, "pair " + std::to_string(i) + " dimension above 32768");
+    }
+}
+
+// Work estimate of one block tile: left-row steps times columns met (times 64 dy rows).
+double tile_cost(uint32_t H, uint32_t W, uint32_t tx, uint32_t ty) {
+    const int dyb = (int)ty * 64 - ((int)H - 1), dxb = (int)tx * 256 - ((int)W - 1);
+    const int dy_hi = std::min(dyb + 63, (int)H - 1), dx_hi = std::min(dxb + 255, (int)W - 1);
+    const int rsteps = std::min((int)H, (int)H + dy_hi) - std::max(0, dyb);
+    const int cols = std::min((int)W, (int)W - dxb) - std::max(0, -dx_hi);
+    return (double)std::max(rsteps, 0) * std::max(cols, 0) * 64.0;
+}
+
+}  // namespace
+
+xc_batch_plan* batch_plan_create(const xc_pair* pairs, uint32_t num_pairs, const xc_options& opt) {
+    check_pairs(pairs, num_pairs);
+    const int kind = opt.precision == XC_PREC_AUTO ? XC_PREC_BF16X3 : (int)opt.precision;
+    if (kind != XC_PREC_BF16X3 && kind != XC_PREC_BF16 && kind != XC_PREC_FP16)
+        fail(XC_ERR_INVALID_ARGUMENT, "batch plans run the MFMA methods (bf16x3, bf16, fp16)");
+    if (opt.border > 512) fail(XC_ERR_INVALID_ARGUMENT, "border threshold above 512");
+    const uint32_t thresh = kind == XC_PREC_BF16X3 ? (opt.border ? opt.border : batch_default_border()) : 0;
+
+    std::vector<batch_pair_desc> descs(num_pairs);
+    std::vector<uint32_t> tables;
+    std::map<std::pair<uint32_t, uint32_t>, std::pair<uint32_t, uint32_t>> table_of;  // size -> (offset, nh)
+    struct costed {
+        double cost;
+        batch_item item;
+    };
+    std::vector<costed> items;
+    for (uint32_t i = 0; i < num_pairs; ++i) {
+        const xc_pair& p = pairs[i];
+        batch_pair_desc& d = descs[i];
+        d.loff = p.left_off;
+        d.roff = p.right_off;
+        d.ooff = p.out_off;
+        d.rows = p.rows;
+        d.cols = p.cols;
+        d.table_off = 0;
+        d.nh = 0;
+        const uint32_t tx_n = batch_tiles_x(p.cols), ty_n = batch_tiles_y(p.rows);
+        for (uint32_t ty = 0; ty < ty_n; ++ty)
+            for (uint32_t tx = 0; tx < tx_n; ++tx)
+                items.push_back({tile_cost(p.rows, p.cols, tx, ty), batch_item{i, 0, (uint16_t)tx, ty}});
+        if (thresh > 1) {
+            auto key = std::make_pair(p.rows, p.cols);
+            auto it = table_of.find(key);
+            if (it == table_of.end()) {
+                std::vector<uint32_t> start;
+                const uint32_t nh = frame_table_host(p.rows, p.cols, thresh, start);
+                it = table_of.emplace(key, std::make_pair((uint32_t)tables.size(), nh)).first;
+                tables.insert(tables.end(), start.begin(), start.end());
+            }
+            d.table_off = it->second.first;
+            d.nh = it->second.second;
+            if (d.nh) {
+                const uint32_t total = tables[d.table_off + d.nh] * 4;
+                for (uint32_t f0 = 0; f0 < total; f0 += batch_frame_chunk())
+                    items.push_back({(double)batch_frame_chunk() * thresh / 4, batch_item{i, 1, 0, f0}});
+            }
+        }
+    }
+    // Largest work first: the dynamic queue then balances the tail with small items.
+    std::stable_sort(items.begin(), items.end(), [](const costed& a, const costed& b) { return a.cost > b.cost; });
+
+    auto plan = std::make_unique<xc_batch_plan>();
+    plan->kind = kind;
+    plan->thresh = thresh;
+    plan->n_items = (uint32_t)items.size();
+    plan->pairs.assign(pairs, pairs + num_pairs);
+    XC_HIP(hipGetDevice(&plan->device));
+    std::vector<batch_item> flat(items.size());
+    for (size_t i = 0; i < items.size(); ++i) flat[i] = items[i].item;
+    if (tables.empty()) tables.push_back(0);
+    auto upload = [](void** dst, const void* src, size_t bytes) {
+        if (hipMalloc(dst, std::max<size_t>(bytes, 16)) != hipSuccess) {
+            (void)hipGetLastError();
+            fail(XC_ERR_NO_MEMORY, "batch plan allocation failed");
+        }
+        if (bytes) XC_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    };
+    upload((void**)&plan->d_items, flat.data(), flat.size() * sizeof(batch_item));
+    upload((void**)&plan->d_pairs, descs.data(), descs.size() * sizeof(batch_pair_desc));
+    upload((void**)&plan->d_tables, tables.data(), tables.size() * sizeof(uint32_t));
+    const uint32_t zero[2] = {0, 0};
+    upload((void**)&plan->d_counters, zero, sizeof(zero));
+    int cus = 0;
+    XC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, plan->device));
+    plan->grid = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)cus * batch_blocks_per_cu(), plan->n_items));
+    plan->lds = batch_lds_bytes(kind);
+    XC_HIP(hipFuncSetAttribute(batch_kernel_ptr(kind), hipFuncAttributeMaxDynamicSharedMemorySize, plan->lds));
+    XC_HIP(hipStreamCreateWithFlags(&plan->capture_stream, hipStreamNonBlocking));
+    return plan.release();
+}
+
+void batch_plan_exec(xc_batch_plan* plan, const float* L, const float* R, float* O, hipStream_t s, bool use_graph) {
+    if (!plan) fail(XC_ERR_INVALID_ARGUMENT, "null batch plan");
+    if (plan->n_items == 0) return;
+    if (!L || !R || !O) fail(XC_ERR_INVALID_ARGUMENT, "null matrix pointer");
+    int dev = 0;
+    XC_HIP(hipGetDevice(&dev));
+    if (dev != plan->device) fail(XC_ERR_INVALID_ARGUMENT, "batch plan used on another device than it was made on");
+    uint32_t n_items = plan->n_items, thresh = plan->thresh;
+    void* args[] = {&plan->d_items, &n_items, &plan->d_pairs, &plan->d_tables, &L, &R, &O, &plan->d_counters, &thresh};
+    record_launch("batch_kernel (persistent)", dim3(plan->grid), dim3(256), plan->lds, 0, 0,
+                  (uint32_t)plan->pairs.size(), 1);
+    if (!use_graph) {
+        XC_HIP(hipLaunchKernel(batch_kernel_ptr(plan->kind), dim3(plan->grid), dim3(256), args, plan->lds, s));
+        return;
+    }
+    if (!plan->exec) {
+        XC_HIP(hipStreamBeginCapture(plan->capture_stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = hipLaunchKernel(batch_kernel_ptr(plan->kind), dim3(plan->grid), dim3(256), args,
+                                              plan->lds, plan->capture_stream);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(plan->capture_stream, &g);
+        XC_HIP(le);
+        XC_HIP(ce);
+        plan->graph = g;
+        size_t count = 1;
+        XC_HIP(hipGraphGetNodes(plan->graph, &plan->node, &count));
+        if (count != 1) fail(XC_ERR_HIP, "unexpected batch graph layout");
+        XC_HIP(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+        plan->L = L;
+        plan->R = R;
+        plan->O = O;
+    } else if (plan->L != L || plan->R != R || plan->O != O) {
+        hipKernelNodeParams p{};
+        XC_HIP(hipGraphKernelNodeGetParams(plan->node, &p));
+        p.kernelParams = args;
+        XC_HIP(hipGraphExecKernelNodeSetParams(plan->exec, plan->node, &p));
+        plan->L = L;
+        plan->R = R;
+        plan->O = O;
+    }
+    XC_HIP(hipGraphLaunch(plan->exec, s));
+}
+
+// One-shot batch: plans are cached by their pair list and options (a few per process).
+void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, const float* R, float* O,
+                   const xc_options& opt, hipStream_t s) {
+    if (opt.precision == XC_PREC_FP32) {
+        // Exact FP32 products: one direct-kernel launch per pair.
+        for (uint32_t i = 0; i < num_pairs; ++i) {
+            family_args a{XC_ONE_TO_ONE, pairs[i].rows, pairs[i].cols, 1, 1, 1};
+            launch_direct_f32(a, L + pairs[i].left_off, R + pairs[i].right_off, O + pairs[i].out_off, opt.split, s);
+        }
+        return;
+    }
+    static std::mutex mu;
+    static std::list<std::pair<std::string, std::unique_ptr<xc_batch_plan>>> cache;
+    std::string key(reinterpret_cast<const char*>(&opt), sizeof(opt));
+    key.append(reinterpret_cast<const char*>(pairs), sizeof(xc_pair) * num_pairs);
+    int dev = 0;
+    XC_HIP(hipGetDevice(&dev));
+    key.append(reinterpret_cast<const char*>(&dev), sizeof(dev));
+    std::lock_guard<std::mutex> lock(mu);
+    xc_batch_plan* plan = nullptr;
+    for (auto it = cache.begin(); it != cache.end(); ++it)
+        if (it->first == key) {
+            cache.splice(cache.begin(), cache, it);
+            plan = cache.front().second.get();
+            break;
+        }
+    if (!plan) {
+        cache.emplace_front(key, std::unique_ptr<xc_batch_plan>(batch_plan_create(pairs, num_pairs, opt)));
+        plan = cache.front().second.get();
+        if (cache.size() > 8) cache.pop_back();
+    }
+    batch_plan_exec(plan, L, R, O, s, true);
+}
+
+}  // namespace xc

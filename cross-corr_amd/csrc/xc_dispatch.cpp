@@ -68,12 +68,4 @@ void run_family_f32(const family_args& a, const float* L, const float* R, float*
     }
 }
 
-void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, const float* R, float* O,
-                   const xc_options& opt, hipStream_t s) {
-    for (uint32_t i = 0; i < num_pairs; ++i) {
-        family_args a{XC_ONE_TO_ONE, pairs[i].rows, pairs[i].cols, 1, 1, 1};
-        run_family_f32(a, L + pairs[i].left_off, R + pairs[i].right_off, O + pairs[i].out_off, opt, s);
-    }
-}
-
 }  // namespace xc

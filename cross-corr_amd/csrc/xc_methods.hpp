@@ -1,6 +1,8 @@
 // Method-level entry points behind the C ABI (one per algorithm family of the library).
 #pragma once
 
+#include <vector>
+
 #include "xc_internal.hpp"
 
 namespace xc {
@@ -18,9 +20,31 @@ void launch_mfma_f32(const family_args& a, const float* L, const float* R, float
                      uint32_t border, uint32_t tile, hipStream_t s);
 int mfma_catalog_size();
 
-// Uneven batch (xc_batch.hip).
+// Uneven batch: work list of the persistent kernel (xc_mfma.hip) and its host plan (xc_batch.cpp).
+struct batch_pair_desc {
+    uint64_t loff, roff, ooff;  // element offsets of the pair's matrices
+    uint32_t rows, cols;
+    uint32_t table_off, nh;     // frame table of the pair's size in the tables buffer
+};
+struct batch_item {
+    uint32_t pair;
+    uint16_t kind;  // 0: MFMA block tile (tx, ty = b); 1: 256 frame outputs starting at b
+    uint16_t tx;
+    uint32_t b;
+};
+uint32_t batch_tiles_x(uint32_t cols);
+uint32_t batch_tiles_y(uint32_t rows);
+uint32_t batch_frame_chunk();
+uint32_t batch_default_border();
+uint32_t frame_table_host(uint32_t rows, uint32_t cols, uint32_t thresh, std::vector<uint32_t>& start);
+const void* batch_kernel_ptr(int kind);
+uint32_t batch_lds_bytes(int kind);
+uint32_t batch_blocks_per_cu();
+
 void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, const float* R, float* O,
                    const xc_options& opt, hipStream_t s);
+xc_batch_plan* batch_plan_create(const xc_pair* pairs, uint32_t num_pairs, const xc_options& opt);
+void batch_plan_exec(xc_batch_plan* plan, const float* L, const float* R, float* O, hipStream_t s, bool use_graph);
 
 // FFT method (xc_fft.hip).
 void run_fft_f32(const family_args& a, const float* L, const float* R, float* O, bool normalize,

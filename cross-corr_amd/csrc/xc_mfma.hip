@@ -208,31 +208,19 @@ __device__ __forceinline__ void chunk_range_dispatch(int mask, f32x16 (&acc)[2][
 // KC: columns per pass; FLUSH: left-row steps summed in `acc` before it is added to the
 // total; KAHAN: compensated add of those sums (else plain); OCC: blocks per CU the register
 // budget is sized for.
+// One block tile (dx tile tx, dy tile ty) of the pair (Lp, Rp) -> Op, both H x W.  Outputs
+// whose overlap area is below skip_area are left to the border-frame code (0: write all).
 template <int KIND, int WX, int WY, int KC, int FLUSH, bool KAHAN, int OCC>
-__global__ __launch_bounds__(256, OCC) void mfma_kernel(const float* __restrict__ L, const float* __restrict__ R,
-                                                      float* __restrict__ O, int family, uint32_t rows,
-                                                      uint32_t cols, uint32_t n, uint32_t m, uint32_t tiles_x,
-                                                      uint32_t tiles_y) {
+__device__ __forceinline__ void mfma_tile(const float* __restrict__ Lp, const float* __restrict__ Rp,
+                                          float* __restrict__ Op, const int H, const int W, const uint32_t tx,
+                                          const uint32_t ty, const uint32_t skip_area, char* smem) {
     using S = split_traits<KIND>;
     constexpr int P = S::planes;
     using G = geom<WX, WY, P, KC>;
     constexpr bool PIPE = OCC == 1;  // two waves per SIMD hide LDS latency without it
-    extern __shared__ __attribute__((aligned(16))) char smem[];
     char* ring = smem;
     char* lbuf = smem + G::RING;
-
-    const int H = (int)rows, W = (int)cols;
     const int SX = 2 * W - 1;
-    uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t tx = bid % tiles_x;
-    bid /= tiles_x;
-    const uint32_t ty = bid % tiles_y;
-    const uint32_t k = bid / tiles_y;
-    uint32_t li, ri;
-    pair_of(family, n, m, k, li, ri);
-    const float* __restrict__ Lp = L + (size_t)li * H * W;
-    const float* __restrict__ Rp = R + (size_t)ri * H * W;
-    float* __restrict__ Op = O + (size_t)k * SX * (2 * H - 1);
 
     const int dxb = (int)tx * G::BX - (W - 1);
     const int dyb = (int)ty * G::BY - (H - 1);
@@ -449,9 +437,30 @@ __global__ __launch_bounds__(256, OCC) void mfma_kernel(const float* __restrict_
 #pragma unroll
             for (int a = 0; a < 16; ++a) {
                 const int dy = dyw + 32 * u + 8 * (a >> 2) + 4 * g + (a & 3);
-                if (dy <= H - 1) Op[(size_t)(dy + H - 1) * SX + (dx + W - 1)] = tot[u][s][a];
+                if (dy <= H - 1 &&
+                    (uint32_t)(H - abs(dy)) * (uint32_t)(W - abs(dx)) >= skip_area)
+                    Op[(size_t)(dy + H - 1) * SX + (dx + W - 1)] = tot[u][s][a];
             }
     }
+}
+
+template <int KIND, int WX, int WY, int KC, int FLUSH, bool KAHAN, int OCC>
+__global__ __launch_bounds__(256, OCC) void mfma_kernel(const float* __restrict__ L, const float* __restrict__ R,
+                                                      float* __restrict__ O, int family, uint32_t rows,
+                                                      uint32_t cols, uint32_t n, uint32_t m, uint32_t tiles_x,
+                                                      uint32_t tiles_y, uint32_t skip_area) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int H = (int)rows, W = (int)cols;
+    uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tx = bid % tiles_x;
+    bid /= tiles_x;
+    const uint32_t ty = bid % tiles_y;
+    const uint32_t k = bid / tiles_y;
+    uint32_t li, ri;
+    pair_of(family, n, m, k, li, ri);
+    mfma_tile<KIND, WX, WY, KC, FLUSH, KAHAN, OCC>(L + (size_t)li * H * W, R + (size_t)ri * H * W,
+                                                   O + (size_t)k * (2 * W - 1) * (2 * H - 1), H, W, tx, ty,
+                                                   skip_area, smem);
 }
 
 // Frame outputs of a (rows, cols, threshold): the overlap sizes (h, w) with h*w < threshold,
@@ -460,6 +469,48 @@ struct frame_table {
     uint32_t nh;
     uint32_t start[kMaxBorder + 1];
 };
+
+// Exact FP32 value of one output (dy, dx) with overlap h x w, computed by one wave: the lanes
+// cover the h*w products (whole rows when w < 64), each lane sums its share and the 64 lane
+// sums are added by a butterfly reduction; lane 0 stores.
+__device__ __forceinline__ void frame_output(const float* __restrict__ Lp, const float* __restrict__ Rp,
+                                             float* __restrict__ Op, int H, int W, int dy, int dx, int h, int w,
+                                             int lane) {
+    const int r0 = max(0, dy), c0 = max(0, dx);
+    int rs, cl, rstep, cstep;
+    if (w >= 64) {
+        rs = 0;
+        cl = lane;
+        rstep = 1;
+        cstep = 64;
+    } else {
+        rstep = 64 / w;
+        rs = lane / w;
+        cl = lane - rs * w;
+        cstep = w;
+        if (rs >= rstep) rs = h;  // idle lane
+    }
+    const float* __restrict__ lp = Lp + (size_t)r0 * W + c0;
+    const float* __restrict__ rp = Rp + (size_t)(r0 - dy) * W + (c0 - dx);
+    float acc = 0.f;
+    for (int r = rs; r < h; r += rstep)
+        for (int c = cl; c < w; c += cstep) acc = fmaf(lp[(size_t)r * W + c], rp[(size_t)r * W + c], acc);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) Op[(size_t)(dy + H - 1) * (2 * W - 1) + (dx + W - 1)] = acc;
+}
+
+// (h, w) of frame index hw from a table with start[0..nh] (largest h with start[h-1] <= hw).
+__device__ __forceinline__ void frame_hw(const uint32_t* start, uint32_t nh, uint32_t hw, int& h, int& w) {
+    int lo = 0, hi = (int)nh - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= hw) lo = mid;
+        else hi = mid - 1;
+    }
+    h = lo + 1;
+    w = (int)(hw - start[lo]) + 1;
+}
 
 // Exact FP32 recomputation of the outputs whose overlap area (H-|dy|)*(W-|dx|) is below the
 // threshold: one wave per (pair, overlap size (h, w), quadrant).  The wave's lanes cover the
@@ -475,48 +526,21 @@ __global__ __launch_bounds__(256) void frame_kernel(const float* __restrict__ L,
     const uint32_t o = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t hw = o >> 2, q = o & 3;
     if (hw >= tab.start[tab.nh]) return;
-    // h: the largest h with start[h-1] <= hw
-    int lo = 0, hi = (int)tab.nh - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (tab.start[mid] <= hw) lo = mid;
-        else hi = mid - 1;
-    }
-    const int h = lo + 1, w = (int)(hw - tab.start[lo]) + 1;
+    int h, w;
+    frame_hw(tab.start, tab.nh, hw, h, w);
     const int dy = (q & 1) ? -(H - h) : (H - h);
     const int dx = (q & 2) ? -(W - w) : (W - w);
     if (((q & 1) && dy == 0) || ((q & 2) && dx == 0)) return;  // the same output as another quadrant
-    const int r0 = max(0, dy), c0 = max(0, dx);
-    // lane -> (first row, column) and row step
-    int rs, cl, rstep, cstep;
-    if (w >= 64) {
-        rs = 0;
-        cl = lane;
-        rstep = 1;
-        cstep = 64;
-    } else {
-        rstep = 64 / w;
-        rs = lane / w;
-        cl = lane - rs * w;
-        cstep = w;
-        if (rs >= rstep) rs = h;  // idle lane
-    }
     for (uint32_t k = blockIdx.y; k < pairs; k += gridDim.y) {
         uint32_t li, ri;
         pair_of(family, n, m, k, li, ri);
-        const float* __restrict__ lp = L + (size_t)li * H * W + (size_t)r0 * W + c0;
-        const float* __restrict__ rp = R + (size_t)ri * H * W + (size_t)(r0 - dy) * W + (c0 - dx);
-        float acc = 0.f;
-        for (int r = rs; r < h; r += rstep)
-            for (int c = cl; c < w; c += cstep) acc = fmaf(lp[(size_t)r * W + c], rp[(size_t)r * W + c], acc);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-        if (lane == 0) O[(size_t)k * SX * SY + (size_t)(dy + H - 1) * SX + (dx + W - 1)] = acc;
+        frame_output(L + (size_t)li * H * W, R + (size_t)ri * H * W, O + (size_t)k * SX * SY, H, W, dy, dx, h, w,
+                     lane);
     }
 }
 
 template <int KIND, int WX, int WY, int KC, int FLUSH, bool KAHAN, int OCC>
-void launch(const family_args& a, const float* L, const float* R, float* O, hipStream_t s) {
+void launch(const family_args& a, const float* L, const float* R, float* O, uint32_t skip_area, hipStream_t s) {
     using G = geom<WX, WY, split_traits<KIND>::planes, KC>;
     const uint32_t SX = 2 * a.cols - 1, SY = 2 * a.rows - 1;
     const uint32_t tiles_x = (SX + G::BX - 1) / G::BX;
@@ -534,7 +558,7 @@ void launch(const family_args& a, const float* L, const float* R, float* O, hipS
                                                                                           : "mfma_kernel<fp16>"),
                   grid, block, G::BYTES, a.rows, a.cols, a.pairs, 1);
     hipLaunchKernelGGL((mfma_kernel<KIND, WX, WY, KC, FLUSH, KAHAN, OCC>), grid, block, G::BYTES, s, L, R, O, a.family, a.rows, a.cols, a.n,
-                       a.m, tiles_x, tiles_y);
+                       a.m, tiles_x, tiles_y, skip_area);
     XC_HIP(hipGetLastError());
 }
 
@@ -561,39 +585,146 @@ void launch_frame_f32(const family_args& a, const float* L, const float* R, floa
     XC_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------- uneven batch
+// Persistent kernel over a work list: MFMA block tiles of pairs of any size, plus (bf16x3)
+// chunks of 256 border-frame outputs.  Tiles skip the frame outputs, so the items write
+// disjoint outputs and need no ordering.  Blocks take items from an atomic counter; the last
+// block to finish resets the counters, so the launch can be replayed (graph) as is.
+namespace {
+
+constexpr int kBWX = 4, kBWY = 1, kBKC = 128, kBFLUSH = 8, kBOCC = 2;
+constexpr bool kBKAHAN = false;
+constexpr uint32_t kFrameChunkOutputs = 256;
+
+template <int KIND>
+__global__ __launch_bounds__(256, kBOCC) void batch_kernel(const batch_item* __restrict__ items, uint32_t n_items,
+                                                          const batch_pair_desc* __restrict__ pairs,
+                                                          const uint32_t* __restrict__ tables,
+                                                          const float* __restrict__ L, const float* __restrict__ R,
+                                                          float* __restrict__ O, uint32_t* counters,
+                                                          uint32_t thresh) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    __shared__ uint32_t s_item;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    while (true) {
+        if (threadIdx.x == 0) s_item = atomicAdd(&counters[0], 1u);
+        __syncthreads();
+        const uint32_t it = s_item;
+        __syncthreads();
+        if (it >= n_items) break;
+        const batch_item bi = items[it];
+        const batch_pair_desc pd = pairs[bi.pair];
+        const int H = (int)pd.rows, W = (int)pd.cols;
+        const float* Lp = L + pd.loff;
+        const float* Rp = R + pd.roff;
+        float* Op = O + pd.ooff;
+        if (bi.kind == 0) {
+            mfma_tile<KIND, kBWX, kBWY, kBKC, kBFLUSH, kBKAHAN, kBOCC>(Lp, Rp, Op, H, W, bi.tx, bi.b,
+                                                                      KIND == XC_PREC_BF16X3 ? thresh : 0, smem);
+        } else {
+            const uint32_t* start = tables + pd.table_off;
+            const uint32_t total = start[pd.nh] * 4;
+            for (uint32_t i = 0; i < kFrameChunkOutputs / 4; ++i) {
+                const uint32_t o = bi.b + wave * (kFrameChunkOutputs / 4) + i;
+                if (o >= total) break;
+                const uint32_t hw = o >> 2, q = o & 3;
+                int h, w;
+                frame_hw(start, pd.nh, hw, h, w);
+                const int dy = (q & 1) ? -(H - h) : (H - h);
+                const int dx = (q & 2) ? -(W - w) : (W - w);
+                if (((q & 1) && dy == 0) || ((q & 2) && dx == 0)) continue;
+                frame_output(Lp, Rp, Op, H, W, dy, dx, h, w, lane);
+            }
+        }
+    }
+    // self-reset for the next launch: the last block out sets both counters back to 0
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(&counters[1], 1u);
+        if (done == gridDim.x - 1) {
+            atomicExch(&counters[0], 0u);
+            atomicExch(&counters[1], 0u);
+        }
+    }
+}
+
+}  // namespace
+
+uint32_t batch_tiles_x(uint32_t cols) { return (2 * cols - 1 + 64 * kBWX - 1) / (64 * kBWX); }
+uint32_t batch_tiles_y(uint32_t rows) { return (2 * rows - 1 + 64 * kBWY - 1) / (64 * kBWY); }
+uint32_t batch_frame_chunk() { return kFrameChunkOutputs; }
+uint32_t batch_default_border() { return kDefaultBorder; }
+
+uint32_t frame_table_host(uint32_t rows, uint32_t cols, uint32_t thresh, std::vector<uint32_t>& start) {
+    start.clear();
+    uint32_t total = 0;
+    for (uint32_t h = 1; h <= rows && h < thresh; ++h) {
+        const uint32_t cnt = std::min<uint32_t>(cols, (thresh + h - 1) / h - 1);
+        if (cnt == 0) break;
+        start.push_back(total);
+        total += cnt;
+    }
+    const uint32_t nh = (uint32_t)start.size();
+    start.push_back(total);
+    return nh;
+}
+
+const void* batch_kernel_ptr(int kind) {
+    switch (kind) {
+    case XC_PREC_BF16X3: return reinterpret_cast<const void*>(&batch_kernel<XC_PREC_BF16X3>);
+    case XC_PREC_BF16: return reinterpret_cast<const void*>(&batch_kernel<XC_PREC_BF16>);
+    case XC_PREC_FP16: return reinterpret_cast<const void*>(&batch_kernel<XC_PREC_FP16>);
+    default: fail(XC_ERR_INVALID_ARGUMENT, "the batch kernel supports bf16x3, bf16 and fp16");
+    }
+}
+
+uint32_t batch_lds_bytes(int kind) {
+    return kind == XC_PREC_BF16X3 ? geom<kBWX, kBWY, 2, kBKC>::BYTES : geom<kBWX, kBWY, 1, kBKC>::BYTES;
+}
+
+uint32_t batch_blocks_per_cu() { return kBOCC; }
+
 // Tile catalog of the split-precision kernel (the runtime replacement of the reference's
 // compile-time template limits, CMakeLists.txt:16-45): xc_options.tile selects an entry,
 // 0 is the default.
 int mfma_catalog_size() { return 6; }
 
+namespace {
+
 template <int KIND>
-void launch_tile(const family_args& a, const float* L, const float* R, float* O, uint32_t tile, hipStream_t s) {
+void launch_tile(const family_args& a, const float* L, const float* R, float* O, uint32_t tile, uint32_t skip_area,
+                 hipStream_t s) {
     switch (tile) {
     case 0:
-    case 4: launch<KIND, 4, 1, 128, 8, false, 2>(a, L, R, O, s); break;
-    case 1: launch<KIND, 4, 1, 128, 4, false, 2>(a, L, R, O, s); break;
-    case 2: launch<KIND, 4, 1, 256, 2, true, 1>(a, L, R, O, s); break;
-    case 3: launch<KIND, 4, 1, 128, 2, true, 1>(a, L, R, O, s); break;
-    case 5: launch<KIND, 4, 1, 64, 8, false, 2>(a, L, R, O, s); break;
-    case 6: launch<KIND, 2, 2, 64, 4, false, 2>(a, L, R, O, s); break;
+    case 4: launch<KIND, 4, 1, 128, 8, false, 2>(a, L, R, O, skip_area, s); break;
+    case 1: launch<KIND, 4, 1, 128, 4, false, 2>(a, L, R, O, skip_area, s); break;
+    case 2: launch<KIND, 4, 1, 256, 2, true, 1>(a, L, R, O, skip_area, s); break;
+    case 3: launch<KIND, 4, 1, 128, 2, true, 1>(a, L, R, O, skip_area, s); break;
+    case 5: launch<KIND, 4, 1, 64, 8, false, 2>(a, L, R, O, skip_area, s); break;
+    case 6: launch<KIND, 2, 2, 64, 4, false, 2>(a, L, R, O, skip_area, s); break;
     default: fail(XC_ERR_INVALID_ARGUMENT, "tile code " + std::to_string(tile) + " is not in the MFMA tile catalog");
     }
 }
+
+}  // namespace
 
 void launch_mfma_f32(const family_args& a, const float* L, const float* R, float* O, int kind, uint32_t border,
                      uint32_t tile, hipStream_t s) {
     if (tile > (uint32_t)mfma_catalog_size())
         fail(XC_ERR_INVALID_ARGUMENT, "tile code " + std::to_string(tile) + " is not in the MFMA tile catalog");
     switch (kind) {
-    case XC_PREC_BF16X3:
-        launch_tile<XC_PREC_BF16X3>(a, L, R, O, tile, s);
-        launch_frame_f32(a, L, R, O, border ? border : kDefaultBorder, s);
+    case XC_PREC_BF16X3: {
+        const uint32_t thresh = border ? border : kDefaultBorder;
+        launch_tile<XC_PREC_BF16X3>(a, L, R, O, tile, thresh, s);
+        launch_frame_f32(a, L, R, O, thresh, s);
         break;
+    }
     case XC_PREC_BF16:
-        launch_tile<XC_PREC_BF16>(a, L, R, O, tile, s);
+        launch_tile<XC_PREC_BF16>(a, L, R, O, tile, 0, s);
         break;
     case XC_PREC_FP16:
-        launch_tile<XC_PREC_FP16>(a, L, R, O, tile, s);
+        launch_tile<XC_PREC_FP16>(a, L, R, O, tile, 0, s);
         break;
     default:
         fail(XC_ERR_INVALID_ARGUMENT, "not an MFMA precision");

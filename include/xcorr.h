@@ -111,6 +111,19 @@ typedef struct xc_pair {
 xc_status xc_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* left, const float* right,
                        float* out, const xc_options* opt, xc_stream_t stream);
 
+/* Reusable batch plan: the work list (MFMA block tiles of every pair, largest first, plus
+ * border-frame chunks for bf16x3) is built and uploaded once; each execution is one launch
+ * of the persistent kernel, captured into a HIP graph on first use and replayed (kernel
+ * arguments are updated in the instantiated graph when the matrix pointers change).
+ * xc_batch_f32 keeps a few such plans cached by pair list.  A plan must not be executed
+ * concurrently with itself (its work counter is shared); destroy synchronises the device. */
+typedef struct xc_batch_plan xc_batch_plan;
+xc_status xc_batch_plan_create_f32(const xc_pair* pairs, uint32_t num_pairs, const xc_options* opt,
+                                   xc_batch_plan** plan);
+xc_status xc_batch_plan_exec_f32(xc_batch_plan* plan, const float* left, const float* right, float* out,
+                                 int use_graph, xc_stream_t stream);
+xc_status xc_batch_plan_destroy(xc_batch_plan* plan);
+
 /* ---- FFT-method building blocks (src/kernels.cuh:7-63) ------------------------------ */
 /* run_scatter (src/kernels.cuh:53-63, src/scatter.cu:15-102): copy src matrices into
  * the top-left (at dst_row, dst_col) of dst matrices and write zeros everywhere else

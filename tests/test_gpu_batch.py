@@ -1,0 +1,90 @@
+"""Uneven batch (n_to_mn-style groups of different sizes) through xc_batch_f32 and reusable
+plans: one persistent launch, captured into a HIP graph and replayed.  Every pair is checked
+against the FP64 oracle (bf16x3: <= 2e-6 normalised; fp32 per-pair direct: <= 1e-6)."""
+import numpy as np
+import pytest
+
+from xc_testutil import batch_oracle_check, uneven_batch, xcorr
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def test_batch_small_uneven_bf16x3():
+    lf, rf, pairs, olen, sizes = uneven_batch(0, 3, sizes=[1, 2, 5, 16, 33, 64, 70, 100, 140, 31])
+    out = torch.full((olen,), float("nan"), device="cuda")
+    xcorr().batch(pairs, dev(lf), dev(rf), out, precision="bf16x3")
+    torch.cuda.synchronize()
+    assert not torch.isnan(out).any()
+    batch_oracle_check(lf, rf, pairs, out.cpu().numpy(), 2e-6)
+
+
+def test_batch_fp32_per_pair():
+    lf, rf, pairs, olen, _ = uneven_batch(0, 4, sizes=[3, 17, 50, 96])
+    out = torch.full((olen,), float("nan"), device="cuda")
+    xcorr().batch(pairs, dev(lf), dev(rf), out, precision="fp32")
+    torch.cuda.synchronize()
+    batch_oracle_check(lf, rf, pairs, out.cpu().numpy(), 1e-6)
+
+
+def test_batch_plan_graph_replays_and_pointer_update():
+    x = xcorr()
+    lf, rf, pairs, olen, _ = uneven_batch(6, 5, lo=16, hi=200)
+    plan = x.BatchPlan(pairs, "bf16x3")
+    L, R = dev(lf), dev(rf)
+    outs = [torch.full((olen,), float("nan"), device="cuda") for _ in range(2)]
+    ref = None
+    for rep in range(4):
+        o = outs[rep % 2]  # alternate output buffers: the graph's kernel arguments are updated
+        o.fill_(float("nan"))
+        plan.run(L, R, o, use_graph=True)
+        torch.cuda.synchronize()
+        # eager work between replays
+        _ = (L * 2.0).sum().item()
+        got = o.cpu().numpy()
+        assert not np.isnan(got).any(), rep
+        if ref is None:
+            batch_oracle_check(lf, rf, pairs, got, 2e-6)
+            ref = got
+        else:
+            np.testing.assert_array_equal(got, ref)
+    # the same plan without the graph
+    o = torch.full((olen,), float("nan"), device="cuda")
+    plan.run(L, R, o, use_graph=False)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(o.cpu().numpy(), ref)
+    plan.close()
+
+
+def test_batch_c4_shape_subset():
+    """The 128-group, 512-pair workload (sizes log-uniform in [16, 512], U[-1,1)): all pairs
+    computed in one launch; the oracle checks the 8 smallest groups and 2 groups near size 128
+    (the FP64 oracle is single-threaded per pair: a 512x512 pair costs it minutes), and the sum
+    rule (sum of a correlation = sum(L) * sum(R)) checks every pair."""
+    lf, rf, pairs, olen, sizes = uneven_batch(128, 2025)
+    out = torch.empty((olen,), device="cuda")
+    xcorr().batch(pairs, dev(lf), dev(rf), out, precision="bf16x3")
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    order = np.argsort(sizes)
+    mid = sorted(range(len(sizes)), key=lambda g: abs(sizes[g] - 128))[:2]
+    groups = list(order[:8]) + mid
+    batch_oracle_check(lf, rf, pairs, got, 2e-6, check_pairs=[4 * g + j for g in groups for j in (0, 3)])
+    for lo, ro, oo, r, c in pairs:
+        s = got[oo:oo + (2 * r - 1) * (2 * c - 1)].astype(np.float64).sum()
+        e = lf[lo:lo + r * c].astype(np.float64).sum() * rf[ro:ro + r * c].astype(np.float64).sum()
+        den = np.abs(lf[lo:lo + r * c]).astype(np.float64).sum() * np.abs(rf[ro:ro + r * c]).astype(np.float64).sum()
+        assert abs(s - e) / den < 2e-6
+
+
+def test_batch_invalid_inputs_rejected():
+    x = xcorr()
+    with pytest.raises(x.XcError):
+        x.BatchPlan([(0, 0, 0, 0, 4)], "bf16x3")
+    with pytest.raises(x.XcError):
+        x.BatchPlan([(0, 0, 0, 4, 4)], "fft64")

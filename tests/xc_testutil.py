@@ -203,3 +203,39 @@ def header_symbols():
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         names += re.findall(r"\b(xc_[a-z0-9_]+)\s*\(", text)
     return sorted(set(names))
+
+
+def uneven_batch(groups, seed, lo=16, hi=512, per_group=4, dist="u11", sizes=None):
+    """Uneven n_to_mn-style batch: `groups` left matrices of square size N_g = round(lo*(hi/lo)^U),
+    U ~ U[0,1) from `seed`, each paired with `per_group` right matrices of its size.
+    Returns (left_flat, right_flat, pairs, out_len, sizes) with pairs as
+    (left_off, right_off, out_off, rows, cols) element offsets into the flat buffers."""
+    rng = np.random.default_rng(seed)
+    if sizes is None:
+        sizes = [int(round(lo * (hi / lo) ** rng.random())) for _ in range(groups)]
+    lefts, rights, pairs = [], [], []
+    loff = roff = ooff = 0
+    for g, n in enumerate(sizes):
+        lefts.append(seeded((n, n), dist, seed * 1000 + 2 * g, np.float32).ravel())
+        for j in range(per_group):
+            rights.append(seeded((n, n), dist, seed * 1000 + 2 * g + 1 + 7919 * (j + 1), np.float32).ravel())
+            pairs.append((loff, roff, ooff, n, n))
+            roff += n * n
+            ooff += (2 * n - 1) ** 2
+        loff += n * n
+    return np.concatenate(lefts), np.concatenate(rights), pairs, ooff, sizes
+
+
+def batch_oracle_check(left_flat, right_flat, pairs, out_flat, tol, check_pairs=None):
+    """Max normalised error of each pair of an uneven batch against the FP64 oracle."""
+    worst = 0.0
+    idx = range(len(pairs)) if check_pairs is None else check_pairs
+    for i in idx:
+        lo, ro, oo, r, c = pairs[i]
+        L = left_flat[lo:lo + r * c].reshape(1, r, c).astype(np.float64)
+        R = right_flat[ro:ro + r * c].reshape(1, r, c).astype(np.float64)
+        got = out_flat[oo:oo + (2 * r - 1) * (2 * c - 1)].reshape(1, 2 * r - 1, 2 * c - 1)
+        err, _ = norm_error(got, oracle_family(0, L, R), oracle_family(0, L, R, absval=True))
+        assert err <= tol, (i, r, c, err)
+        worst = max(worst, err)
+    return worst
