@@ -43,6 +43,29 @@ class Pair(ctypes.Structure):
                 ("rows", ctypes.c_uint32), ("cols", ctypes.c_uint32)]
 
 
+class Workload(ctypes.Structure):
+    _fields_ = [("family", ctypes.c_uint32), ("rows", ctypes.c_uint32), ("cols", ctypes.c_uint32),
+                ("n", ctypes.c_uint32), ("m", ctypes.c_uint32)]
+
+
+class TrainOptions(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_uint32), ("lr", ctypes.c_float), ("weight_decay", ctypes.c_float),
+                ("seed", ctypes.c_uint32), ("hidden", ctypes.c_uint32), ("use_graph", ctypes.c_int),
+                ("reserved", ctypes.c_uint32 * 2)]
+
+
+class MlpShape(ctypes.Structure):
+    _fields_ = [("layers", ctypes.c_uint32), ("dims", ctypes.c_uint32 * 8)]
+
+    @classmethod
+    def of(cls, dims):
+        s = cls()
+        s.layers = len(dims) - 1
+        for i, d in enumerate(dims):
+            s.dims[i] = d
+        return s
+
+
 class LaunchDesc(ctypes.Structure):
     _fields_ = [("kernel", ctypes.c_char * 96), ("grid", ctypes.c_uint32 * 3), ("block", ctypes.c_uint32 * 3),
                 ("lds_bytes", ctypes.c_uint32), ("rows", ctypes.c_uint32), ("cols", ctypes.c_uint32),
@@ -74,6 +97,26 @@ SIGNATURES = {
     "xc_run_ccn_shuffle_f32": (_st, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _vp]),
     "xc_run_ccn_shuffle_one_to_many_multirow_both_multimat_right_f32":
         (_st, [_vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _vp]),
+    "xc_num_candidates": (_u32, []),
+    "xc_candidate": (_st, [_u32, ctypes.POINTER(Options), ctypes.c_char_p, ctypes.c_size_t]),
+    "xc_workload_features": (_st, [ctypes.POINTER(Workload), ctypes.POINTER(ctypes.c_float), _u32,
+                                   ctypes.POINTER(ctypes.c_uint32)]),
+    "xc_autotune_f32": (_st, [_i32, _vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, ctypes.POINTER(ctypes.c_float),
+                              ctypes.POINTER(ctypes.c_uint32), _vp]),
+    "xc_autotune_clear": (_st, []),
+    "xc_cost_model_load": (_st, [ctypes.c_char_p]),
+    "xc_cost_model_predict": (_st, [ctypes.POINTER(Workload), ctypes.POINTER(ctypes.c_float)]),
+    "xc_select": (_st, [ctypes.POINTER(Workload), ctypes.POINTER(Options), ctypes.POINTER(ctypes.c_int)]),
+    "xc_cost_model_fit": (_st, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(TrainOptions), _u32]),
+    "xc_mlp_num_params": (_u64, [ctypes.POINTER(MlpShape)]),
+    "xc_mlp_work_size": (_u64, [ctypes.POINTER(MlpShape), _u32]),
+    "xc_mlp_init": (_st, [ctypes.POINTER(MlpShape), _u64, _vp]),
+    "xc_mlp_forward_f32": (_st, [ctypes.POINTER(MlpShape), _vp, _vp, _u32, _vp, _vp, _vp]),
+    "xc_mlp_loss_grad_f32": (_st, [ctypes.POINTER(MlpShape), _vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp]),
+    "xc_mlp_adam_f32": (_st, [_u64, _vp, _vp, _vp, _vp, _u32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                              ctypes.c_float, ctypes.c_float, _vp]),
+    "xc_mlp_train_f32": (_st, [ctypes.POINTER(MlpShape), _vp, _vp, _vp, _u32, ctypes.POINTER(TrainOptions), _vp,
+                               _vp]),
     "xc_last_error": (ctypes.c_char_p, []),
     "xc_last_launch": (ctypes.POINTER(LaunchDesc), []),
     "xc_version": (ctypes.c_char_p, []),
@@ -211,3 +254,69 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+# ---- method selection (autotuner + cost model) ------------------------------------------
+SELECT_SOURCES = {0: "model", 1: "tuned", 2: "none"}
+
+
+def candidates():
+    """[(name, precision, tile)] of every candidate XC_PREC_AUTO chooses from."""
+    out = []
+    for i in range(lib().xc_num_candidates()):
+        o = Options()
+        name = ctypes.create_string_buffer(32)
+        check(lib().xc_candidate(i, ctypes.byref(o), name, 32))
+        out.append((name.value.decode(), o.precision, o.tile))
+    return out
+
+
+def _workload(family, rows, cols, n, m):
+    fam = FAMILIES[family] if isinstance(family, str) else family
+    return Workload(fam, rows, cols, n, m)
+
+
+def features(family, rows, cols, n, m):
+    buf = (ctypes.c_float * 64)()
+    cnt = ctypes.c_uint32()
+    check(lib().xc_workload_features(ctypes.byref(_workload(family, rows, cols, n, m)), buf, 64, ctypes.byref(cnt)))
+    return [buf[i] for i in range(cnt.value)]
+
+
+def autotune(family, left, right, out, reps=3, stream=None):
+    """Exhaustive autotune on torch device tensors: (ms per candidate, index of the fastest)."""
+    fam = FAMILIES[family] if isinstance(family, str) else family
+    n, rows, cols = left.shape
+    m = right.shape[0]
+    nc = lib().xc_num_candidates()
+    ms = (ctypes.c_float * nc)()
+    best = ctypes.c_uint32()
+    fn = lib().xc_autotune_f32
+    check(fn(fam, _ptr(left), _ptr(right), _ptr(out), rows, cols, n, m, reps, ms, ctypes.byref(best),
+             _stream(stream)))
+    return [ms[i] for i in range(nc)], best.value
+
+
+def cost_model_load(path=None):
+    check(lib().xc_cost_model_load(None if path is None else str(path).encode()))
+
+
+def predict(family, rows, cols, n, m):
+    nc = lib().xc_num_candidates()
+    ms = (ctypes.c_float * nc)()
+    check(lib().xc_cost_model_predict(ctypes.byref(_workload(family, rows, cols, n, m)), ms))
+    return [ms[i] for i in range(nc)]
+
+
+def select(family, rows, cols, n, m):
+    """(options, source) that XC_PREC_AUTO would use without autotuning."""
+    o = Options()
+    src = ctypes.c_int()
+    check(lib().xc_select(ctypes.byref(_workload(family, rows, cols, n, m)), ctypes.byref(o), ctypes.byref(src)))
+    return o, SELECT_SOURCES.get(src.value, src.value)
+
+
+def cost_model_fit(timing_csv, model_path, epochs=3000, lr=3e-3, weight_decay=0.0, seed=1, hidden=32,
+                   use_graph=True, holdout_mod=0):
+    o = TrainOptions(epochs, lr, weight_decay, seed, hidden, int(use_graph))
+    check(lib().xc_cost_model_fit(str(timing_csv).encode(), str(model_path).encode(), ctypes.byref(o), holdout_mod))

@@ -3,11 +3,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <string>
 
 #include "xc_internal.hpp"
 #include "xc_methods.hpp"
+#include "xc_mlp.hpp"
 
 namespace {
 
@@ -143,7 +145,7 @@ xc_status xc_batch_plan_exec_f32(xc_batch_plan* plan, const float* left, const f
 }
 
 xc_status xc_batch_plan_destroy(xc_batch_plan* plan) {
-    return guarded([&] { delete plan; });
+    return guarded([&] { xc::batch_plan_destroy(plan); });
 }
 
 xc_status xc_scatter_f32(const float* src, float* dst, uint32_t src_rows, uint32_t src_cols, uint32_t num_matrices,
@@ -278,6 +280,146 @@ xc_status xc_run_ccn_shuffle_one_to_many_multirow_both_multimat_right_f32(
         check_family(XC_ONE_TO_MANY, rows, cols, 1, num_right_matrices, left, right, out);
         xc::family_args a{XC_ONE_TO_MANY, rows, cols, 1, num_right_matrices, num_right_matrices};
         xc::launch_direct_f32(a, left, right, out, 0, (hipStream_t)stream);
+    });
+}
+
+// ---- method selection and the cost model's MLP
+uint32_t xc_num_candidates(void) { return xc::num_candidates(); }
+
+xc_status xc_candidate(uint32_t i, xc_options* opt, char* name, size_t name_len) {
+    return guarded([&] {
+        if (!opt) xc::fail(XC_ERR_INVALID_ARGUMENT, "null options pointer");
+        *opt = xc::candidate(i);
+        if (name && name_len) {
+            const char* p = opt->precision == XC_PREC_FP32 ? "fp32" : opt->precision == XC_PREC_BF16X3 ? "bf16x3" : "fft64";
+            std::snprintf(name, name_len, "%s/%u", p, opt->tile);
+        }
+    });
+}
+
+xc_status xc_workload_features(const xc_workload* w, float* features, uint32_t capacity, uint32_t* count) {
+    return guarded([&] {
+        if (!w || !features) xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        const uint32_t n = xc::workload_features(*w, features, capacity);
+        if (count) *count = n;
+    });
+}
+
+xc_status xc_autotune_f32(xc_family family, const float* left, const float* right, float* out, uint32_t rows,
+                          uint32_t cols, uint32_t n, uint32_t m, uint32_t reps, float* ms, uint32_t* best,
+                          xc_stream_t stream) {
+    return guarded([&] {
+        check_family(family, rows, cols, n, m, left, right, out);
+        xc::family_args a{family, rows, cols, n, m, (uint32_t)xc::num_results(family, n, m)};
+        const uint32_t b = xc::autotune_f32(a, left, right, out, reps, ms, (hipStream_t)stream);
+        if (best) *best = b;
+    });
+}
+
+xc_status xc_autotune_clear(void) {
+    return guarded([&] { xc::autotune_clear(); });
+}
+
+xc_status xc_cost_model_load(const char* path) {
+    return guarded([&] { xc::cost_model_load(path); });
+}
+
+xc_status xc_cost_model_predict(const xc_workload* w, float* ms) {
+    return guarded([&] {
+        if (!w || !ms) xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        xc::cost_model_predict(*w, ms);
+    });
+}
+
+xc_status xc_select(const xc_workload* w, xc_options* chosen, int* source) {
+    return guarded([&] {
+        if (!w || !chosen) xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        xc_options c{};
+        const int src = xc::select_method(*w, c);
+        *chosen = c;
+        if (source) *source = src;
+    });
+}
+
+xc_status xc_cost_model_fit(const char* timing_csv, const char* model_path, const xc_train_options* opt,
+                            uint32_t holdout_mod) {
+    return guarded([&] {
+        xc_train_options o{};
+        o.epochs = 3000;
+        o.lr = 3e-3f;
+        o.seed = 1;
+        o.use_graph = 1;
+        if (opt) o = *opt;
+        xc::cost_model_fit(timing_csv, model_path, o, holdout_mod);
+    });
+}
+
+uint64_t xc_mlp_num_params(const xc_mlp_shape* shape) {
+    try {
+        xc::mlp_check_shape(shape);
+        return xc::mlp_num_params(*shape);
+    } catch (...) {
+        return 0;
+    }
+}
+
+uint64_t xc_mlp_work_size(const xc_mlp_shape* shape, uint32_t batch) {
+    try {
+        xc::mlp_check_shape(shape);
+        return xc::mlp_work_size(*shape, batch);
+    } catch (...) {
+        return 0;
+    }
+}
+
+xc_status xc_mlp_init(const xc_mlp_shape* shape, uint64_t seed, float* params_host) {
+    return guarded([&] {
+        xc::mlp_check_shape(shape);
+        if (!params_host) xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        xc::mlp_init(*shape, seed, params_host);
+    });
+}
+
+xc_status xc_mlp_forward_f32(const xc_mlp_shape* shape, const float* params, const float* x, uint32_t batch,
+                             float* y, float* work, xc_stream_t stream) {
+    return guarded([&] {
+        xc::mlp_check_shape(shape);
+        if (!params || !x || !y || !work) xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        if (batch == 0) return;
+        xc::mlp_forward(*shape, params, x, batch, y, work, (hipStream_t)stream);
+    });
+}
+
+xc_status xc_mlp_loss_grad_f32(const xc_mlp_shape* shape, const float* params, const float* x, const float* target,
+                               const float* mask, uint32_t batch, float* loss, float* grad, float* work,
+                               xc_stream_t stream) {
+    return guarded([&] {
+        xc::mlp_check_shape(shape);
+        if (!params || !x || !target || !mask || !loss || !grad || !work)
+            xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        if (batch == 0) xc::fail(XC_ERR_INVALID_ARGUMENT, "empty batch");
+        xc::mlp_loss_grad(*shape, params, x, target, mask, batch, loss, grad, work, (hipStream_t)stream);
+    });
+}
+
+xc_status xc_mlp_adam_f32(uint64_t count, float* params, const float* grad, float* m, float* v, uint32_t step,
+                          float lr, float beta1, float beta2, float eps, float weight_decay, xc_stream_t stream) {
+    return guarded([&] {
+        if (!params || !grad || !m || !v) xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        if (step == 0) xc::fail(XC_ERR_INVALID_ARGUMENT, "Adam steps are numbered from 1");
+        if (count == 0) return;
+        xc::mlp_adam(count, params, grad, m, v, step, lr, beta1, beta2, eps, weight_decay, (hipStream_t)stream);
+    });
+}
+
+xc_status xc_mlp_train_f32(const xc_mlp_shape* shape, const float* x_host, const float* target_host,
+                           const float* mask_host, uint32_t batch, const xc_train_options* opt, float* params_host,
+                           float* loss_host) {
+    return guarded([&] {
+        xc::mlp_check_shape(shape);
+        if (!x_host || !target_host || !mask_host || !params_host || !opt)
+            xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        xc::mlp_train(*shape, x_host, target_host, mask_host, batch, *opt, params_host, loss_host);
     });
 }
 

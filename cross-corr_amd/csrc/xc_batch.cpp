@@ -193,6 +193,8 @@ void batch_plan_exec(xc_batch_plan* plan, const float* L, const float* R, float*
     XC_HIP(hipGraphLaunch(plan->exec, s));
 }
 
+void batch_plan_destroy(xc_batch_plan* plan) { delete plan; }
+
 // One-shot batch: plans are cached by their pair list and options (a few per process).
 void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, const float* R, float* O,
                    const xc_options& opt, hipStream_t s) {

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "xc_methods.hpp"
+#include "xc_mlp.hpp"
 
 namespace xc {
 
@@ -49,6 +50,8 @@ void run_family_f32(const family_args& a, const float* L, const float* R, float*
                     hipStream_t s) {
     switch (opt.precision) {
     case XC_PREC_AUTO:
+        run_family_f32(a, L, R, O, auto_options_f32(a, L, R, O, s), s);
+        return;
     case XC_PREC_FP32:
         launch_direct_f32(a, L, R, O, opt.split, s);
         return;

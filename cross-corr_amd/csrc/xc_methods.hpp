@@ -44,6 +44,7 @@ uint32_t batch_blocks_per_cu();
 void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, const float* R, float* O,
                    const xc_options& opt, hipStream_t s);
 xc_batch_plan* batch_plan_create(const xc_pair* pairs, uint32_t num_pairs, const xc_options& opt);
+void batch_plan_destroy(xc_batch_plan* plan);  // runs the plan's destructor
 void batch_plan_exec(xc_batch_plan* plan, const float* L, const float* R, float* O, hipStream_t s, bool use_graph);
 
 // FFT method (xc_fft.hip).

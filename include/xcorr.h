@@ -179,6 +179,93 @@ xc_status xc_run_ccn_shuffle_one_to_many_multirow_both_multimat_right_f32(
     uint32_t warps_per_thread_block, uint32_t shifts_per_thread_right_matrix,
     uint32_t right_matrices_per_thread, uint32_t left_rows_per_iteration, xc_stream_t stream);
 
+/* ---- method selection: exhaustive autotuner and learned cost model ------------------
+ * No counterpart in the reference, whose users pick an algorithm by name
+ * (src/main.cpp:368-409, src/run_args.cpp:87-150).  XC_PREC_AUTO resolves a call to one
+ * CANDIDATE, a concrete (precision, tile) whose results meet the FP32 accuracy bar
+ * (normalised error <= 2e-6): the FP32 direct kernel, each bf16x3 MFMA tile of the tile
+ * catalog, and the FFT method in FP64.  In this order AUTO uses
+ *   1. the process's autotune table (filled by xc_autotune_f32 and by step 3),
+ *   2. the cost model, if one is loaded and the matrix size lies in its training range:
+ *      a small MLP that predicts every candidate's time from the workload's features,
+ *   3. otherwise an exhaustive autotune on the call's own buffers (every candidate is run
+ *      and timed; this synchronises the stream once per candidate).
+ * The model is read on first use from $XC_COST_MODEL, else from tuning/cost_model.txt next
+ * to libxcorr.so.  $XC_AUTO=autotune skips step 2; $XC_AUTO=model never autotunes (it falls
+ * back to bf16x3 with the default tile when the model cannot answer). */
+typedef struct xc_workload {
+    uint32_t family, rows, cols, n, m;
+} xc_workload;
+
+typedef enum xc_select_source {
+    XC_SELECT_MODEL = 0,    /* the cost model's prediction */
+    XC_SELECT_TUNED = 1,    /* the autotune table */
+    XC_SELECT_NONE = 2      /* neither can answer */
+} xc_select_source;
+
+uint32_t xc_num_candidates(void);
+/* Options of candidate i (and its name "precision/tile", e.g. "bf16x3/4", if name != NULL). */
+xc_status xc_candidate(uint32_t i, xc_options* opt, char* name, size_t name_len);
+/* The cost model's input features of a workload (count written to *count). */
+xc_status xc_workload_features(const xc_workload* w, float* features, uint32_t capacity, uint32_t* count);
+/* Exhaustive autotune: runs every candidate on the given buffers (one untimed warm-up, then
+ * `reps` runs timed with HIP events on `stream`), writes the mean ms of each candidate
+ * (INFINITY for one that failed) to ms[xc_num_candidates()], the fastest to *best, and
+ * records the winner in the autotune table.  `out` holds the last candidate's results. */
+xc_status xc_autotune_f32(xc_family family, const float* left, const float* right, float* out, uint32_t rows,
+                          uint32_t cols, uint32_t n, uint32_t m, uint32_t reps, float* ms, uint32_t* best,
+                          xc_stream_t stream);
+xc_status xc_autotune_clear(void);
+/* Load a cost-model file (NULL: the default path).  XC_ERR_UNSUPPORTED if it is missing
+ * or was trained for another candidate list. */
+xc_status xc_cost_model_load(const char* path);
+/* Predicted ms of every candidate (host computation; no GPU needed). */
+xc_status xc_cost_model_predict(const xc_workload* w, float* ms);
+/* What AUTO picks without autotuning: writes the options and returns the source in *source. */
+xc_status xc_select(const xc_workload* w, xc_options* chosen, int* source);
+/* Fit the cost model to a timing table written by the tuning collector (CSV: family, rows,
+ * cols, n, m, then one column of ms per candidate) on the GPU and write the model file.
+ * Rows i with holdout_mod > 0 and i % holdout_mod == 0 are left out of training. */
+typedef struct xc_train_options {
+    uint32_t epochs;
+    float lr;
+    float weight_decay;
+    uint32_t seed;
+    uint32_t hidden;    /* width of the two hidden layers (cost_model_fit only) */
+    int use_graph;      /* replay each epoch from a HIP graph */
+    uint32_t reserved[2];
+} xc_train_options;
+xc_status xc_cost_model_fit(const char* timing_csv, const char* model_path, const xc_train_options* opt,
+                            uint32_t holdout_mod);
+
+/* The cost model's network on the GPU: an MLP with `layers` fully connected layers,
+ * dims[0] inputs ... dims[layers] outputs, tanh on hidden layers.  Parameters are one flat
+ * FP32 array, layer by layer W (dims[l+1] x dims[l], row-major) then b (dims[l+1]).
+ * Pointers other than the *_host ones are device pointers; x, y, target and mask are
+ * batch-major; `work` holds xc_mlp_work_size floats. */
+typedef struct xc_mlp_shape {
+    uint32_t layers;
+    uint32_t dims[8];
+} xc_mlp_shape;
+uint64_t xc_mlp_num_params(const xc_mlp_shape* shape);
+uint64_t xc_mlp_work_size(const xc_mlp_shape* shape, uint32_t batch);
+/* Glorot-uniform weights and zero biases from a seed (host array). */
+xc_status xc_mlp_init(const xc_mlp_shape* shape, uint64_t seed, float* params_host);
+xc_status xc_mlp_forward_f32(const xc_mlp_shape* shape, const float* params, const float* x, uint32_t batch,
+                             float* y, float* work, xc_stream_t stream);
+/* loss = sum mask*(y - target)^2 / sum mask, and d loss / d params into grad. */
+xc_status xc_mlp_loss_grad_f32(const xc_mlp_shape* shape, const float* params, const float* x, const float* target,
+                               const float* mask, uint32_t batch, float* loss, float* grad, float* work,
+                               xc_stream_t stream);
+/* One Adam step (decoupled weight decay), step numbered from 1. */
+xc_status xc_mlp_adam_f32(uint64_t count, float* params, const float* grad, float* m, float* v, uint32_t step,
+                          float lr, float beta1, float beta2, float eps, float weight_decay, xc_stream_t stream);
+/* Full-batch training from host arrays; params_host is the start point and receives the
+ * result; loss_host (may be NULL) receives the loss of every epoch. */
+xc_status xc_mlp_train_f32(const xc_mlp_shape* shape, const float* x_host, const float* target_host,
+                           const float* mask_host, uint32_t batch, const xc_train_options* opt, float* params_host,
+                           float* loss_host);
+
 /* ---- diagnostics (src/kernel_args.hpp:9-33) ----------------------------------------- */
 typedef struct xc_launch_desc {
     char kernel[96];

@@ -1,0 +1,127 @@
+"""Method selection on the host (no GPU): the cost model's features, the model file, host
+inference against a numpy forward pass of the same file, and what the trained model gains
+on held-out workloads of the committed timing table over the same choice made without it."""
+import ctypes
+import pathlib
+
+import numpy as np
+import pytest
+
+import mlp_ref
+from xc_testutil import REPO, xcorr
+
+TUNING = REPO / "cross-corr_amd" / "tuning"
+MODEL = TUNING / "cost_model.txt"
+TIMINGS = TUNING / "timings.csv"
+FAM = ["one_to_one", "one_to_many", "n_to_mn", "n_to_m"]
+
+
+def py_features(family, rows, cols, n, m):
+    """Restatement of the feature definition (xc_select.cpp workload_features)."""
+    k = {0: 1, 1: m, 2: m, 3: n * m}[family]
+    transforms = {0: 3, 1: 1 + 2 * m, 2: n + 2 * m, 3: n + m + k}[family]
+    lr, lc = np.log2(rows), np.log2(cols)
+    pow2 = lambda v: 1.0 if v & (v - 1) == 0 else 0.0
+    return [lr, lc, np.log2(n), np.log2(m), np.log2(k), np.log2(k) + 2 * lr + 2 * lc,
+            np.log2(transforms) + 2 + lr + lc, pow2(2 * rows), pow2(2 * cols)] + [float(family == i) for i in range(4)]
+
+
+def parse_model(path):
+    tok = open(path).read().split()
+    assert tok[0] == "xcorr-cost-model" and tok[1] == "1"
+    i = 2
+    out = {}
+    while i < len(tok) and tok[i] != "trained_rows":
+        key, n = tok[i], int(tok[i + 1])
+        vals = tok[i + 2:i + 2 + n]
+        out[key] = vals if key == "candidates" else np.array(vals, np.float64)
+        i += 2 + n
+    return out
+
+
+def read_table(path):
+    lines = open(path).read().strip().splitlines()
+    head = lines[0].split(",")
+    w, t = [], []
+    for line in lines[1:]:
+        c = line.split(",")
+        w.append(tuple(int(v) for v in c[:5]))
+        t.append([float(v) for v in c[5:]])
+    return head[5:], w, np.array(t)
+
+
+@pytest.fixture(scope="module")
+def x():
+    x = xcorr()
+    x.cost_model_load(str(MODEL))
+    return x
+
+
+def test_features_match_definition(x):
+    for fam, r, c, n, m in [(0, 16, 16, 1, 1), (1, 256, 256, 1, 64), (2, 100, 37, 3, 12), (3, 64, 64, 16, 64)]:
+        np.testing.assert_allclose(x.features(FAM[fam], r, c, n, m), py_features(fam, r, c, n, m), rtol=1e-6)
+
+
+def test_host_inference_matches_numpy_forward(x):
+    mdl = parse_model(MODEL)
+    dims = [int(v) for v in mdl["dims"]]
+    assert mdl["candidates"] == [c[0] for c in x.candidates()]
+    for fam, r, c, n, m in [(1, 256, 256, 1, 64), (3, 64, 64, 16, 64), (0, 40, 900, 1, 1)]:
+        f = (np.array(py_features(fam, r, c, n, m)) - mdl["feat_mean"]) / mdl["feat_std"]
+        y, _ = mlp_ref.forward(dims, mdl["params"], f[None, :])
+        want = 2.0 ** (y[0] * mdl["out_std"] + mdl["out_mean"])
+        np.testing.assert_allclose(x.predict(FAM[fam], r, c, n, m), want, rtol=1e-5)
+
+
+def test_select_sources(x):
+    o, src = x.select("one_to_many", 256, 256, 1, 64)
+    assert src == "model"
+    ms = x.predict("one_to_many", 256, 256, 1, 64)
+    assert x.candidates()[int(np.argmin(ms))][1:] == (o.precision, o.tile)
+    # a matrix size far outside the training range is not answered by the model
+    assert x.select("one_to_one", 8192, 8192, 1, 1)[1] == "none"
+
+
+def test_model_file_is_checked(x, tmp_path):
+    text = MODEL.read_text()
+    bad = tmp_path / "other_candidates.txt"
+    bad.write_text(text.replace("fft64/0", "fft64/9", 1))
+    with pytest.raises(x.XcError) as e:
+        x.cost_model_load(str(bad))
+    assert e.value.status == 4
+    trunc = tmp_path / "truncated.txt"
+    trunc.write_text(text[: len(text) // 2])
+    with pytest.raises(x.XcError):
+        x.cost_model_load(str(trunc))
+    with pytest.raises(x.XcError):
+        x.cost_model_load(str(tmp_path / "missing.txt"))
+    x.cost_model_load(str(MODEL))  # the good model is back
+
+
+def test_trained_model_gain_on_held_out_workloads(x):
+    """Held-out rows (index % 5 == 0, never seen in training) of the committed timing
+    table: total and geometric-mean time of the model's picks against (a) the best single
+    fixed method chosen on the training rows -- the choice without the network -- and
+    (b) exhaustive autotuning (the measured best, an upper bound)."""
+    cands, w, t = read_table(TIMINGS)
+    assert cands == [c[0] for c in x.candidates()]
+    hold = np.arange(len(w)) % 5 == 0
+    train = ~hold
+    fixed = int(np.argmin(np.log(t[train]).sum(0)))  # best fixed method on the training rows
+    picks = []
+    for (fam, r, c, n, m) in np.array(w)[hold]:
+        o, src = x.select(FAM[fam], int(r), int(c), int(n), int(m))
+        assert src == "model"
+        picks.append([i for i, cc in enumerate(x.candidates()) if cc[1:] == (o.precision, o.tile)][0])
+    th = t[hold]
+    model_t = th[np.arange(len(th)), picks]
+    best_t = th.min(1)
+    fixed_t = th[:, fixed]
+    gm = lambda v: float(np.exp(np.log(v).mean()))
+    report = {"model_total_ms": model_t.sum(), "fixed_total_ms": fixed_t.sum(), "exhaustive_total_ms": best_t.sum(),
+              "model_regret_gm": gm(model_t / best_t), "fixed_regret_gm": gm(fixed_t / best_t),
+              "fixed_method": cands[fixed], "held_out": int(hold.sum())}
+    print(report)
+    assert report["model_regret_gm"] < report["fixed_regret_gm"]
+    assert report["model_total_ms"] <= report["fixed_total_ms"] * 1.02
+    assert report["model_regret_gm"] < 1.15
