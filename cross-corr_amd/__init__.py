@@ -117,6 +117,8 @@ SIGNATURES = {
                               ctypes.c_float, ctypes.c_float, _vp]),
     "xc_mlp_train_f32": (_st, [ctypes.POINTER(MlpShape), _vp, _vp, _vp, _u32, ctypes.POINTER(TrainOptions), _vp,
                                _vp]),
+    "xc_kernel_timing": (_st, [_i32]),
+    "xc_kernel_times": (_st, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "xc_last_error": (ctypes.c_char_p, []),
     "xc_last_launch": (ctypes.POINTER(LaunchDesc), []),
     "xc_version": (ctypes.c_char_p, []),
@@ -320,3 +322,19 @@ def cost_model_fit(timing_csv, model_path, epochs=3000, lr=3e-3, weight_decay=0.
                    use_graph=True, holdout_mod=0):
     o = TrainOptions(epochs, lr, weight_decay, seed, hidden, int(use_graph))
     check(lib().xc_cost_model_fit(str(timing_csv).encode(), str(model_path).encode(), ctypes.byref(o), holdout_mod))
+
+
+def kernel_timing(enable=True):
+    check(lib().xc_kernel_timing(int(enable)))
+
+
+def kernel_times():
+    """{kernel name: (launches, total ms)} since the last call (synchronises)."""
+    need = ctypes.c_size_t()
+    buf = ctypes.create_string_buffer(1 << 16)
+    check(lib().xc_kernel_times(buf, len(buf), ctypes.byref(need)))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms = line.rsplit(" ", 2)
+        out[name] = (int(cnt), float(ms))
+    return out

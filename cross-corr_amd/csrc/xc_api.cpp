@@ -104,7 +104,9 @@ xc_status xc_cross_corr_f64(xc_family family, const double* left, const double* 
         if (o.precision != XC_PREC_AUTO && o.precision != XC_PREC_FP32 && o.precision != XC_PREC_FFT64)
             xc::fail(XC_ERR_INVALID_ARGUMENT, "double data supports the direct (FP64) and FFT64 methods only");
         xc::family_args a{family, rows, cols, n, m, (uint32_t)xc::num_results(family, n, m)};
-        if (o.precision == XC_PREC_FFT64)
+        if (o.precision == XC_PREC_FFT64 && o.tile == 1)
+            xc::run_fftk_f64(a, left, right, out, (hipStream_t)stream);
+        else if (o.precision == XC_PREC_FFT64)
             xc::run_fft_f64(a, left, right, out, true, (hipStream_t)stream);
         else
             xc::launch_direct_f64(a, left, right, out, o.split, (hipStream_t)stream);
@@ -420,6 +422,21 @@ xc_status xc_mlp_train_f32(const xc_mlp_shape* shape, const float* x_host, const
         if (!x_host || !target_host || !mask_host || !params_host || !opt)
             xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
         xc::mlp_train(*shape, x_host, target_host, mask_host, batch, *opt, params_host, loss_host);
+    });
+}
+
+xc_status xc_kernel_timing(int enable) {
+    return guarded([&] { xc::timing_enable(enable != 0); });
+}
+
+xc_status xc_kernel_times(char* buf, size_t len, size_t* needed) {
+    return guarded([&] {
+        const std::string r = xc::timing_report();
+        if (needed) *needed = r.size() + 1;
+        if (buf && len) {
+            std::strncpy(buf, r.c_str(), len - 1);
+            buf[len - 1] = 0;
+        }
     });
 }
 

@@ -162,6 +162,7 @@ void batch_plan_exec(xc_batch_plan* plan, const float* L, const float* R, float*
     record_launch("batch_kernel (persistent)", dim3(plan->grid), dim3(256), plan->lds, 0, 0,
                   (uint32_t)plan->pairs.size(), 1);
     if (!use_graph) {
+        timed tb("batch_kernel", s);
         XC_HIP(hipLaunchKernel(batch_kernel_ptr(plan->kind), dim3(plan->grid), dim3(256), args, plan->lds, s));
         return;
     }
@@ -190,6 +191,7 @@ void batch_plan_exec(xc_batch_plan* plan, const float* L, const float* R, float*
         plan->R = R;
         plan->O = O;
     }
+    timed tg("batch_graph", s);
     XC_HIP(hipGraphLaunch(plan->exec, s));
 }
 

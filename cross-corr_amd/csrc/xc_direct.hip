@@ -227,11 +227,13 @@ void launch_direct(const family_args& a, const T* L, const T* R, T* O, uint32_t 
     if (atomic) {
         const uint64_t count = (uint64_t)a.pairs * SX * SY;
         uint32_t zb = (uint32_t)std::min<uint64_t>((count + 255) / 256, 8192);
+        timed tz("zero_kernel", s);
         hipLaunchKernelGGL(zero_kernel<T>, dim3(zb), dim3(256), 0, s, O, count);
     }
     dim3 grid((uint32_t)nblocks), block(256);
     record_launch(sizeof(T) == 4 ? "direct_kernel<f32>" : "direct_kernel<f64>", grid, block, lds, a.rows, a.cols,
                   a.pairs, split);
+    timed td("direct_kernel", s);
     hipLaunchKernelGGL((direct_kernel<T, TY, CW>), grid, block, lds, s, L, R, O, a.family, a.rows, a.cols, a.n,
                        a.m, split, tiles_x, tiles_y, len, atomic);
     XC_HIP(hipGetLastError());

@@ -151,23 +151,31 @@ void run_fft(const family_args& a, const Ti* L, const Ti* R, Ti* O, bool normali
     C* ospec = all_pairs ? (C*)workspace(WS_FFT_OUT, (uint64_t)k * sarea * sizeof(C), s) : spec + nl * sarea;
 
     uint64_t tot = (uint64_t)nl * parea;
-    hipLaunchKernelGGL((scatter_kernel<Ti, Tt>), dim3(grid_for(tot)), dim3(256), 0, s, L, pad, H, W, PR, PC, 0u, 0u,
-                       tot);
-    tot = (uint64_t)nr * parea;
-    hipLaunchKernelGGL((scatter_kernel<Ti, Tt>), dim3(grid_for(tot)), dim3(256), 0, s, R, pad + nl * parea, H, W,
-                       PR, PC, 0u, 0u, tot);
+    {
+        timed t0("fft_scatter", s);
+        hipLaunchKernelGGL((scatter_kernel<Ti, Tt>), dim3(grid_for(tot)), dim3(256), 0, s, L, pad, H, W, PR, PC, 0u,
+                           0u, tot);
+        tot = (uint64_t)nr * parea;
+        hipLaunchKernelGGL((scatter_kernel<Ti, Tt>), dim3(grid_for(tot)), dim3(256), 0, s, R, pad + nl * parea, H, W,
+                           PR, PC, 0u, 0u, tot);
+    }
     XC_HIP(hipGetLastError());
 
     hipfftHandle fwd = get_plan(PR, PC, nl + nr, fft_types<Tt>::r2c);
     fft_check(hipfftSetStream(fwd, s), "hipfftSetStream");
-    fft_types<Tt>::fwd(fwd, pad, spec);
+    {
+        timed t1("rocfft_forward", s);
+        fft_types<Tt>::fwd(fwd, pad, spec);
+    }
 
     if (all_pairs) {
         tot = (uint64_t)k * sarea;
+        timed t2("fft_hadamard", s);
         hipLaunchKernelGGL(hadamard_n_to_m_kernel<C>, dim3(grid_for(tot)), dim3(256), 0, s, spec, spec + nl * sarea,
                            ospec, sarea, nr, tot);
     } else {
         tot = (uint64_t)nr * sarea;
+        timed t2("fft_hadamard", s);
         hipLaunchKernelGGL(hadamard_kernel<C>, dim3(grid_for(tot)), dim3(256), 0, s, spec, ospec, sarea, nl, tot);
     }
     XC_HIP(hipGetLastError());
@@ -177,10 +185,14 @@ void run_fft(const family_args& a, const Ti* L, const Ti* R, Ti* O, bool normali
     fft_check(hipfftSetStream(inv, s), "hipfftSetStream");
     const bool direct_out = !normalize && std::is_same<Ti, Tt>::value;
     Tt* res = direct_out ? (Tt*)O : (Tt*)workspace(WS_FFT_RES, (uint64_t)k * parea * sizeof(Tt), s);
-    fft_types<Tt>::inv(inv, ospec, res);
+    {
+        timed t3("rocfft_inverse", s);
+        fft_types<Tt>::inv(inv, ospec, res);
+    }
 
     if (normalize) {
         tot = (uint64_t)k * (PR - 1) * (PC - 1);
+        timed t4("fft_normalize", s);
         hipLaunchKernelGGL((normalize_kernel<Tt, Ti>), dim3(grid_for(tot)), dim3(256), 0, s, res, O, PR, PC, tot);
     } else if (!direct_out) {
         tot = (uint64_t)k * parea;  // plain precision conversion of the unnormalised output

@@ -31,6 +31,26 @@ inline void hip_check(hipError_t e, const char* what) {
 void record_launch(const char* kernel, dim3 grid, dim3 block, uint32_t lds, uint32_t rows, uint32_t cols,
                    uint32_t pairs, uint32_t split);
 
+// Kernel timing (xc_kernel_timing / xc_kernel_times): while enabled, a `timed` scope around
+// a launch brackets it with two HIP events recorded on the launch's stream.
+bool timing_enabled();
+void timing_enable(bool on);
+std::string timing_report();  // "name count total_ms" lines; clears the records
+void timing_mark(const char* name, hipStream_t s, int phase);
+struct timed {
+    const char* name;
+    hipStream_t s;
+    bool on;
+    timed(const char* n, hipStream_t st) : name(n), s(st), on(timing_enabled()) {
+        if (on) timing_mark(name, s, 0);
+    }
+    ~timed() {
+        if (on) timing_mark(name, s, 1);
+    }
+    timed(const timed&) = delete;
+    timed& operator=(const timed&) = delete;
+};
+
 // k-th result of a family -> (left index, right index); see the table in xcorr.h.
 __host__ __device__ inline void pair_of(int family, uint32_t n, uint32_t m, uint32_t k, uint32_t& l,
                                         uint32_t& r) {

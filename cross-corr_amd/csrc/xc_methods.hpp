@@ -9,7 +9,10 @@ namespace xc {
 
 // Grow-only device workspace, one per purpose; released at process exit.
 void* workspace(int slot, size_t bytes, hipStream_t s);
-enum : int { WS_FFT_PAD = 0, WS_FFT_SPEC = 1, WS_FFT_OUT = 2, WS_FFT_RES = 3, WS_SPLIT = 4, WS_BATCH = 5 };
+enum : int {
+    WS_FFT_PAD = 0, WS_FFT_SPEC = 1, WS_FFT_OUT = 2, WS_FFT_RES = 3, WS_SPLIT = 4, WS_BATCH = 5,
+    WS_FK_SR = 6, WS_FK_SL = 7, WS_FK_LHAT = 8, WS_FK_Z = 9
+};
 
 // Dispatch of xc_cross_corr_f32 on opt.precision (xc_precision).
 void run_family_f32(const family_args& a, const float* L, const float* R, float* O, const xc_options& opt,
@@ -52,6 +55,9 @@ void run_fft_f32(const family_args& a, const float* L, const float* R, float* O,
                  bool fp64_transform, hipStream_t s);
 void run_fft_f64(const family_args& a, const double* L, const double* R, double* O, bool normalize,
                  hipStream_t s);
+// FFT method on the library's own FP64 FFT kernels (xc_fftk.hip): rows, cols <= 2048.
+void run_fftk_f32(const family_args& a, const float* L, const float* R, float* O, hipStream_t s);
+void run_fftk_f64(const family_args& a, const double* L, const double* R, double* O, hipStream_t s);
 
 void check_scatter(const void* src, const void* dst, uint32_t src_rows, uint32_t src_cols, uint32_t dst_rows,
                    uint32_t dst_cols, uint32_t dst_row, uint32_t dst_col);

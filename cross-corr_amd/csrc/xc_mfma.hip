@@ -560,6 +560,7 @@ void launch(const family_args& a, const float* L, const float* R, float* O, uint
     record_launch(KIND == XC_PREC_BF16X3 ? "mfma_kernel<bf16x3>" : (KIND == XC_PREC_BF16 ? "mfma_kernel<bf16>"
                                                                                           : "mfma_kernel<fp16>"),
                   grid, block, G::BYTES, a.rows, a.cols, a.pairs, 1);
+    timed tm("mfma_kernel", s);
     hipLaunchKernelGGL((mfma_kernel<KIND, WX, WY, KC, FLUSH, KAHAN, OCC>), grid, block, G::BYTES, s, L, R, O, a.family, a.rows, a.cols, a.n,
                        a.m, tiles_x, tiles_y, skip_area);
     XC_HIP(hipGetLastError());
@@ -583,6 +584,7 @@ void launch_frame_f32(const family_args& a, const float* L, const float* R, floa
     tab.start[nh] = total;
     const uint32_t blocks = (uint32_t)(((uint64_t)total * 4 + 3) / 4);  // one wave per (h, w, quadrant)
     dim3 grid(blocks, std::min<uint32_t>(a.pairs, 65535));
+    timed tf("frame_kernel", s);
     hipLaunchKernelGGL(frame_kernel, grid, dim3(256), 0, s, L, R, O, a.family, a.rows, a.cols, a.n, a.m, a.pairs,
                        tab);
     XC_HIP(hipGetLastError());

@@ -214,7 +214,7 @@ void write_list(std::ostream& out, const char* key, const std::vector<double>& v
 
 }  // namespace
 
-uint32_t num_candidates() { return 2 + (uint32_t)mfma_catalog_size(); }
+uint32_t num_candidates() { return 3 + (uint32_t)mfma_catalog_size(); }
 
 xc_options candidate(uint32_t i) {
     xc_options o{};
@@ -224,8 +224,9 @@ xc_options candidate(uint32_t i) {
     } else if (i <= nt) {
         o.precision = XC_PREC_BF16X3;
         o.tile = i;
-    } else if (i == nt + 1) {
+    } else if (i == nt + 1 || i == nt + 2) {
         o.precision = XC_PREC_FFT64;
+        o.tile = i - nt - 1;  // 0: rocFFT, 1: own FFT kernels
     } else {
         fail(XC_ERR_INVALID_ARGUMENT, "candidate index out of range");
     }

@@ -60,7 +60,8 @@ typedef enum xc_family {
  *                   the border threshold are recomputed exactly in FP32
  *  XC_PREC_BF16     MFMA, single bf16 product (fast, ~1e-3 relative error)
  *  XC_PREC_FP16     MFMA, single fp16 product (inputs must fit fp16 range)
- *  XC_PREC_FFT64    FFT method computed in FP64 (rocFFT D2Z/Z2D), results in FP32
+ *  XC_PREC_FFT64    FFT method computed in FP64, results in FP32; tile 0: rocFFT D2Z/Z2D,
+ *                   tile 1: this library's fused FP64 FFT kernels (rows, cols <= 2048)
  *  XC_PREC_FFT32    FFT method in FP32 (rocFFT R2C/C2R); large error at low-overlap outputs
  *  XC_PREC_AUTO     the library picks (cost model / autotuned table)           */
 typedef enum xc_precision {
@@ -274,6 +275,14 @@ typedef struct xc_launch_desc {
     uint32_t lds_bytes;
     uint32_t rows, cols, pairs, split;
 } xc_launch_desc;
+
+/* Kernel timing for measurement: while enabled, every kernel (or rocFFT execution) the
+ * library enqueues is bracketed by two HIP events recorded on its own stream.
+ * xc_kernel_times synchronises on them and writes one line per kernel name,
+ * "name launches total_ms", then forgets them; *needed gets the full text size.
+ * Not for launches under stream capture. */
+xc_status xc_kernel_timing(int enable);
+xc_status xc_kernel_times(char* buf, size_t len, size_t* needed);
 
 const char* xc_last_error(void);             /* thread-local; "" if none */
 const xc_launch_desc* xc_last_launch(void);  /* thread-local; last launch this thread enqueued */
