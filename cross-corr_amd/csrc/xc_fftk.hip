@@ -25,6 +25,7 @@
 
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "xc_methods.hpp"
@@ -32,11 +33,9 @@
 namespace xc {
 namespace {
 
-constexpr int kFT = 256;                       // threads per block
-constexpr int kLogMaxP = 12;                   // longest transform: 4096
-constexpr int kSlab = 1 << kLogMaxP;           // complex values per block
-constexpr int kSlabPad = kSlab + kSlab / 8;    // with one pad slot per 8
-constexpr uint32_t kLdsBytes = kSlabPad * 16;  // 73,728 B
+constexpr int kFT = 256;              // threads per block
+constexpr int kLogMaxP = 12;          // longest transform: 4096 (the twiddle table's length)
+constexpr int kSlab = 1 << kLogMaxP;  // twiddle table entries
 
 __device__ __forceinline__ int pad(int j) { return j + (j >> 3); }
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -88,30 +87,50 @@ __device__ __forceinline__ void dft(double2 (&u)[R]) {
     }
 }
 
+// Barrier between the read and write phases of a pass.  In WAVE mode every transform of the
+// slab is owned by one wave for all passes, so ordering that wave's own LDS accesses is enough.
+template <bool WAVE>
+__device__ __forceinline__ void slab_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        __syncthreads();
+    }
+}
+
 // One Stockham radix-R pass over T transforms of length P = 2^logP stored back to back in
-// buf (padded index).  logp: log2 of the product of the radices already applied.
-// Butterfly i of a transform reads elements i + r*P/R and writes (i - k)*R + k + r*p,
-// k = i mod p, after multiplying input r by exp(-2 pi i r k / (p R)).
-template <int R, int LOGR>
+// buf (padded index), T * P <= 2^LOGS.  logp: log2 of the product of the radices already
+// applied.  Butterfly i of a transform reads elements i + r*P/R and writes (i - k)*R + k + r*p,
+// k = i mod p, after multiplying input r by exp(-2 pi i r k / (p R)).  Butterflies go to
+// threads block-strided, or (WAVE, T a multiple of 4) as one contiguous range of whole
+// transforms per wave.
+template <int R, int LOGR, int LOGS, bool WAVE>
 __device__ __forceinline__ void fft_pass(double2* buf, int logP, int T, int logp, const double2* __restrict__ tw) {
-    constexpr int MAXB = kSlab / R / kFT;
+    constexpr int MAXB = (1 << LOGS) / R / kFT;
+    static_assert(MAXB >= 1, "slab too small for the radix");
     const int logPR = logP - LOGR;
     const int nb = T << logPR;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int per_wave = nb >> 2;
     double2 u[MAXB][R];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
-        const int b = threadIdx.x + q * kFT;
-        if (b < nb) {
+        const bool ok = WAVE ? lane + 64 * q < per_wave : (int)threadIdx.x + kFT * q < nb;
+        const int b = WAVE ? wave * per_wave + lane + 64 * q : (int)threadIdx.x + kFT * q;
+        if (ok) {
             const int base = (b >> logPR) << logP, i = b & ((1 << logPR) - 1);
 #pragma unroll
             for (int r = 0; r < R; ++r) u[q][r] = buf[pad(base + i + (r << logPR))];
         }
     }
-    __syncthreads();
+    slab_sync<WAVE>();
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
-        const int b = threadIdx.x + q * kFT;
-        if (b < nb) {
+        const bool ok = WAVE ? lane + 64 * q < per_wave : (int)threadIdx.x + kFT * q < nb;
+        const int b = WAVE ? wave * per_wave + lane + 64 * q : (int)threadIdx.x + kFT * q;
+        if (ok) {
             const int base = (b >> logPR) << logP, i = b & ((1 << logPR) - 1);
             const int k = i & ((1 << logp) - 1);
             if (logp > 0) {
@@ -129,28 +148,31 @@ __device__ __forceinline__ void fft_pass(double2* buf, int logP, int T, int logp
             for (int r = 0; r < R; ++r) buf[pad(base + j + (r << logp))] = u[q][r];
         }
     }
-    __syncthreads();
+    slab_sync<WAVE>();
 }
 
-// Forward FFT of T transforms of length 2^logP (T << logP <= 4096) in buf; natural order.
+// Forward FFT of T transforms of length 2^logP in buf; natural order.  The caller has
+// synchronised the block before; the block is synchronised on return.
+template <int LOGS, bool WAVE>
 __device__ void fft_slab(double2* buf, int logP, int T, const double2* __restrict__ tw) {
     int logp = 0;
     const int rem = logP % 3;
     if (rem == 1) {
-        fft_pass<2, 1>(buf, logP, T, logp, tw);
+        fft_pass<2, 1, LOGS, WAVE>(buf, logP, T, logp, tw);
         logp += 1;
     } else if (rem == 2) {
-        fft_pass<4, 2>(buf, logP, T, logp, tw);
+        fft_pass<4, 2, LOGS, WAVE>(buf, logP, T, logp, tw);
         logp += 2;
     }
     while (logp < logP) {
-        fft_pass<8, 3>(buf, logP, T, logp, tw);
+        fft_pass<8, 3, LOGS, WAVE>(buf, logP, T, logp, tw);
         logp += 3;
     }
+    if constexpr (WAVE) __syncthreads();
 }
 
 // rows_fwd: block (matrix, group of 2T rows).
-template <typename Tin>
+template <typename Tin, int LOGS, bool WAVE>
 __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X, uint32_t H, uint32_t W, int logPc,
                                                        int logT, uint32_t groups, double2* __restrict__ S,
                                                        const double2* __restrict__ tw) {
@@ -167,7 +189,7 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
         buf[pad(idx)] = make_double2(re, im);
     }
     __syncthreads();
-    fft_slab(buf, logPc, T, tw);
+    fft_slab<LOGS, WAVE>(buf, logPc, T, tw);
     const uint32_t Kc = (uint32_t)(Pc / 2 + 1);
     const int rows2 = 2 << logT;
     double2* Sm = S + (size_t)mat * Kc * H;
@@ -192,6 +214,7 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
 // cols: block (item, group of C columns).  mode 0: item = left matrix l, writes its full
 // column spectra Lhat[l][k][rho].  mode 1: item = pair q0 + item, writes
 // Z[item][k][rho] = Pr * IFFT(Lhat[l] * conj(FFT(S[r])))  along rho.
+template <int LOGS, bool WAVE>
 __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S, double2* __restrict__ Lhat,
                                                    double2* __restrict__ Z, int mode, int family, uint32_t n,
                                                    uint32_t m, uint32_t q0, uint32_t H, uint32_t Kc, int logPr,
@@ -208,7 +231,7 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
         buf[pad(idx)] = (k < Kc && rho < H) ? Sm[(size_t)k * H + rho] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_slab(buf, logPr, C, tw);
+    fft_slab<LOGS, WAVE>(buf, logPr, C, tw);
     double2* Ll = Lhat + (size_t)l * Kc * Pr;
     if (mode == 0) {
         for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
@@ -224,7 +247,7 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
         buf[pad(idx)] = cmul(cconj(lh), buf[pad(idx)]);
     }
     __syncthreads();
-    fft_slab(buf, logPr, C, tw);
+    fft_slab<LOGS, WAVE>(buf, logPr, C, tw);
     double2* Zq = Z + (size_t)item * Kc * Pr;
     for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
         const uint32_t k = k0 + (idx >> logPr);
@@ -234,7 +257,7 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
 
 // rows_inv: block (pair item, group of 2T output rows oy).  Output row oy = dy + H - 1 reads
 // Z row rho = dy mod Pr; output column ox = dx + W - 1 takes circular column dx mod Pc.
-template <typename Tout>
+template <typename Tout, int LOGS, bool WAVE>
 __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict__ Z, Tout* __restrict__ O,
                                                        uint32_t q0, uint32_t H, uint32_t W, int logPc, int logPr,
                                                        int logT, uint32_t groups, double scale,
@@ -274,7 +297,7 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
         }
         __syncthreads();
     }
-    fft_slab(buf, logPc, T, tw);
+    fft_slab<LOGS, WAVE>(buf, logPc, T, tw);
     Tout* Oq = O + (size_t)(q0 + item) * SY * SX;
     const int rows2 = 2 << logT;
     for (int idx = threadIdx.x; idx < rows2 * (int)SX; idx += kFT) {
@@ -334,6 +357,41 @@ const double2* twiddles() {
     return tables[dev];
 }
 
+struct slab_shape {
+    int logS;      // log2 of the slab (values per block)
+    bool wave;     // transforms owned by single waves
+    int logT;      // log2 of the transforms per block
+    uint32_t lds;  // dynamic LDS bytes
+};
+
+slab_shape shape_for(int logP, uint32_t want) {
+    slab_shape v{};
+    v.logS = logP <= 11 ? 11 : 12;
+    v.wave = logP <= 9;  // T >= 4 then fits the 2048-value slab
+    int lt = v.wave ? 2 : 0;
+    while (lt + logP < v.logS && (1u << lt) < want) ++lt;
+    v.logT = lt;
+    v.lds = (uint32_t)(((1u << (logP + lt)) * 9 / 8) * sizeof(double2));
+    return v;
+}
+
+template <typename F>
+void with_shape(const slab_shape& v, F&& f) {
+    if (v.logS == 12)
+        f(std::integral_constant<int, 12>{}, std::false_type{});
+    else if (v.wave)
+        f(std::integral_constant<int, 11>{}, std::true_type{});
+    else
+        f(std::integral_constant<int, 11>{}, std::false_type{});
+}
+
+template <typename K>
+void allow_lds(K kernel, uint32_t bytes) {
+    if (bytes > 65536)
+        XC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   bytes));
+}
+
 template <typename Tio>
 void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, hipStream_t s) {
     const uint32_t H = a.rows, W = a.cols;
@@ -346,36 +404,15 @@ void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, hipStrea
     const uint32_t pairs = a.pairs;
     const double2* tw = twiddles();
 
-    static bool attr_done = false;
-    static std::mutex attr_mu;
-    {
-        std::lock_guard<std::mutex> lock(attr_mu);
-        if (!attr_done) {
-            XC_HIP(hipFuncSetAttribute((const void*)rows_fwd_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       kLdsBytes));
-            XC_HIP(hipFuncSetAttribute((const void*)rows_fwd_kernel<double>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
-            XC_HIP(hipFuncSetAttribute((const void*)cols_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
-            XC_HIP(hipFuncSetAttribute((const void*)rows_inv_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       kLdsBytes));
-            XC_HIP(hipFuncSetAttribute((const void*)rows_inv_kernel<double>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes));
-            attr_done = true;
-        }
-    }
-    // Slab shapes: transforms per block (powers of two, slab <= 4096 values).
-    auto slab_T = [](int logP, uint32_t want) {
-        int lt = 0;
-        while (lt + logP < kLogMaxP && (1u << lt) < want) ++lt;
-        return lt;
-    };
-    const int logT1 = slab_T(logPc, (H + 1) / 2);        // rows_fwd: 2T input rows per block
-    const int logC = slab_T(logPr, Kc);                   // cols: C columns per block
-    const int logT3 = slab_T(logPc, H);                   // rows_inv: 2T output rows per block (2H-1 rows)
+    // Slab shape per kernel: transforms per block, and the variant (slab of 2048 values with
+    // wave-owned transforms for P <= 512; 2048 block-wide for P = 1024, 2048; 4096 for 4096).
+    const slab_shape v1 = shape_for(logPc, (H + 1) / 2);  // rows_fwd: 2T input rows per block
+    const slab_shape vc = shape_for(logPr, Kc);           // cols: C columns per block
+    const slab_shape v3 = shape_for(logPc, H);            // rows_inv: 2T output rows per block
+    const int logT1 = v1.logT, logC = vc.logT, logT3 = v3.logT;
     const uint32_t g1 = (H + (2u << logT1) - 1) >> (logT1 + 1);
     const uint32_t cg = (Kc + (1u << logC) - 1) >> logC;
     const uint32_t g3 = (2 * H - 1 + (2u << logT3) - 1) >> (logT3 + 1);
-    auto lds = [](int logP, int logT) { return (uint32_t)(((1u << (logP + logT)) * 9 / 8) * sizeof(double2)); };
 
     const size_t s_bytes = sizeof(double2) * (size_t)Kc * H;
     const size_t col_bytes = sizeof(double2) * (size_t)Kc * Pr;
@@ -386,35 +423,50 @@ void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, hipStrea
     const uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(pairs, (size_t(2) << 30) / col_bytes));
     double2* Zw = static_cast<double2*>(workspace(WS_FK_Z, col_bytes * chunk, s));
 
+    auto rows_fwd = [&](const Tio* X, uint32_t nmat, double2* S) {
+        with_shape(v1, [&](auto LS, auto WV) {
+            auto k = rows_fwd_kernel<Tio, decltype(LS)::value, decltype(WV)::value>;
+            allow_lds(k, v1.lds);
+            hipLaunchKernelGGL(k, dim3(nmat * g1), dim3(kFT), v1.lds, s, X, H, W, logPc, logT1, g1, S, tw);
+        });
+    };
+    auto cols = [&](const double2* S, double2* Zout, int mode, uint32_t items, uint32_t q0) {
+        with_shape(vc, [&](auto LS, auto WV) {
+            auto k = cols_kernel<decltype(LS)::value, decltype(WV)::value>;
+            allow_lds(k, vc.lds);
+            hipLaunchKernelGGL(k, dim3(items * cg), dim3(kFT), vc.lds, s, S, LH, Zout, mode, a.family, a.n, a.m, q0,
+                               H, Kc, logPr, logC, cg, tw);
+        });
+    };
     {
         timed t0("fftk_rows_fwd_left", s);
-        hipLaunchKernelGGL(rows_fwd_kernel<Tio>, dim3(nl * g1), dim3(kFT), lds(logPc, logT1), s, L, H, W, logPc,
-                           logT1, g1, SL, tw);
+        rows_fwd(L, nl, SL);
     }
     {
         timed t1("fftk_cols_left", s);
-        hipLaunchKernelGGL(cols_kernel, dim3(nl * cg), dim3(kFT), lds(logPr, logC), s, (const double2*)SL, LH,
-                           (double2*)nullptr, 0, a.family, a.n, a.m, 0u, H, Kc, logPr, logC, cg, tw);
+        cols(SL, nullptr, 0, nl, 0);
     }
     {
         timed t2("fftk_rows_fwd", s);
-        hipLaunchKernelGGL(rows_fwd_kernel<Tio>, dim3(nr * g1), dim3(kFT), lds(logPc, logT1), s, R, H, W, logPc,
-                           logT1, g1, SR, tw);
+        rows_fwd(R, nr, SR);
     }
     const double scale = 1.0 / ((double)Pr * (double)Pc);
     for (uint32_t q0 = 0; q0 < pairs; q0 += chunk) {
         const uint32_t nq = std::min(chunk, pairs - q0);
         {
             timed t3("fftk_cols", s);
-            hipLaunchKernelGGL(cols_kernel, dim3(nq * cg), dim3(kFT), lds(logPr, logC), s, (const double2*)SR, LH,
-                               Zw, 1, a.family, a.n, a.m, q0, H, Kc, logPr, logC, cg, tw);
+            cols(SR, Zw, 1, nq, q0);
         }
         timed t4("fftk_rows_inv", s);
-        hipLaunchKernelGGL(rows_inv_kernel<Tio>, dim3(nq * g3), dim3(kFT), lds(logPc, logT3), s, (const double2*)Zw, O,
-                           q0, H, W, logPc, logPr, logT3, g3, scale, tw);
+        with_shape(v3, [&](auto LS, auto WV) {
+            auto k = rows_inv_kernel<Tio, decltype(LS)::value, decltype(WV)::value>;
+            allow_lds(k, v3.lds);
+            hipLaunchKernelGGL(k, dim3(nq * g3), dim3(kFT), v3.lds, s, (const double2*)Zw, O, q0, H, W, logPc, logPr,
+                               logT3, g3, scale, tw);
+        });
     }
     XC_HIP(hipGetLastError());
-    record_launch("fftk_rows_inv", dim3(g3), dim3(kFT), lds(logPc, logT3), H, W, pairs, 1);
+    record_launch("fftk_rows_inv", dim3(g3), dim3(kFT), v3.lds, H, W, pairs, 1);
 }
 
 }  // namespace
