@@ -68,7 +68,7 @@ def kernel_bytes(name, H, W, n, m, k):
     SY, SX = 2 * H - 1, 2 * W - 1
     out = k * SY * SX * 4
     if name == "fftk_rows_fwd":
-        return m * H * W * 4 + m * Kc * H * 16
+        return (n + m) * (H * W * 4 + Kc * H * 16)
     if name == "fftk_cols":
         return m * Kc * H * 16 + n * Kc * Pr * 16 + k * Kc * Pr * 16
     if name == "fftk_rows_inv":

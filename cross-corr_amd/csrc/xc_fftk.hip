@@ -37,7 +37,12 @@ constexpr int kFT = 256;              // threads per block
 constexpr int kLogMaxP = 12;          // longest transform: 4096 (the twiddle table's length)
 constexpr int kSlab = 1 << kLogMaxP;  // twiddle table entries
 
-__device__ __forceinline__ int pad(int j) { return j + (j >> 3); }
+// LDS slot of element j of transform t (length 2^logP): one pad slot per 8 values, and a
+// transform stride of 9P/8 + 1 slots, so that lanes touching the same j of consecutive
+// transforms, or j = 8i, hit different banks.
+__device__ __forceinline__ int at(int t, int j, int logP) {
+    return t * ((1 << logP) + (1 << logP) / 8 + 1) + j + (j >> 3);
+}
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -100,8 +105,8 @@ __device__ __forceinline__ void slab_sync() {
     }
 }
 
-// One Stockham radix-R pass over T transforms of length P = 2^logP stored back to back in
-// buf (padded index), T * P <= 2^LOGS.  logp: log2 of the product of the radices already
+// One Stockham radix-R pass over T transforms of length P = 2^logP in buf (slots `at`),
+// T * P <= 2^LOGS.  logp: log2 of the product of the radices already
 // applied.  Butterfly i of a transform reads elements i + r*P/R and writes (i - k)*R + k + r*p,
 // k = i mod p, after multiplying input r by exp(-2 pi i r k / (p R)).  Butterflies go to
 // threads block-strided, or (WAVE, T a multiple of 4) as one contiguous range of whole
@@ -120,9 +125,9 @@ __device__ __forceinline__ void fft_pass(double2* buf, int logP, int T, int logp
         const bool ok = WAVE ? lane + 64 * q < per_wave : (int)threadIdx.x + kFT * q < nb;
         const int b = WAVE ? wave * per_wave + lane + 64 * q : (int)threadIdx.x + kFT * q;
         if (ok) {
-            const int base = (b >> logPR) << logP, i = b & ((1 << logPR) - 1);
+            const int t = b >> logPR, i = b & ((1 << logPR) - 1);
 #pragma unroll
-            for (int r = 0; r < R; ++r) u[q][r] = buf[pad(base + i + (r << logPR))];
+            for (int r = 0; r < R; ++r) u[q][r] = buf[at(t, i + (r << logPR), logP)];
         }
     }
     slab_sync<WAVE>();
@@ -131,7 +136,7 @@ __device__ __forceinline__ void fft_pass(double2* buf, int logP, int T, int logp
         const bool ok = WAVE ? lane + 64 * q < per_wave : (int)threadIdx.x + kFT * q < nb;
         const int b = WAVE ? wave * per_wave + lane + 64 * q : (int)threadIdx.x + kFT * q;
         if (ok) {
-            const int base = (b >> logPR) << logP, i = b & ((1 << logPR) - 1);
+            const int t = b >> logPR, i = b & ((1 << logPR) - 1);
             const int k = i & ((1 << logp) - 1);
             if (logp > 0) {
                 const double2 w1 = tw[k << (kLogMaxP - logp - LOGR)];
@@ -145,7 +150,7 @@ __device__ __forceinline__ void fft_pass(double2* buf, int logP, int T, int logp
             dft<R>(u[q]);
             const int j = ((i - k) << LOGR) + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[pad(base + j + (r << logp))] = u[q][r];
+            for (int r = 0; r < R; ++r) buf[at(t, j + (r << logp), logP)] = u[q][r];
         }
     }
     slab_sync<WAVE>();
@@ -171,22 +176,27 @@ __device__ void fft_slab(double2* buf, int logP, int T, const double2* __restric
     if constexpr (WAVE) __syncthreads();
 }
 
-// rows_fwd: block (matrix, group of 2T rows).
+// rows_fwd: block (matrix, group of 2T rows); matrices 0..nl-1 are the left ones (into SL),
+// the rest the right ones (into SR).
 template <typename Tin, int LOGS, bool WAVE>
-__global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X, uint32_t H, uint32_t W, int logPc,
-                                                       int logT, uint32_t groups, double2* __restrict__ S,
-                                                       const double2* __restrict__ tw) {
+__global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ XL, const Tin* __restrict__ XR,
+                                                       uint32_t nl, uint32_t H, uint32_t W, int logPc, int logT,
+                                                       uint32_t groups, double2* __restrict__ SL,
+                                                       double2* __restrict__ SR, const double2* __restrict__ tw) {
     extern __shared__ double2 buf[];
-    const uint32_t mat = blockIdx.x / groups, g = blockIdx.x - mat * groups;
+    const uint32_t matg = blockIdx.x / groups, g = blockIdx.x - matg * groups;
+    const bool left = matg < nl;
+    const uint32_t mat = left ? matg : matg - nl;
     const int Pc = 1 << logPc, T = 1 << logT;
     const uint32_t r0 = g << (logT + 1);
-    const Tin* Xm = X + (size_t)mat * H * W;
+    const Tin* Xm = (left ? XL : XR) + (size_t)mat * H * W;
+    double2* S = left ? SL : SR;
     for (int idx = threadIdx.x; idx < (T << logPc); idx += kFT) {
         const int t = idx >> logPc, c = idx & (Pc - 1);
         const uint32_t a = r0 + 2 * t, b = a + 1;
         const double re = (a < H && (uint32_t)c < W) ? (double)Xm[(size_t)a * W + c] : 0.0;
         const double im = (b < H && (uint32_t)c < W) ? (double)Xm[(size_t)b * W + c] : 0.0;
-        buf[pad(idx)] = make_double2(re, im);
+        buf[at(t, c, logPc)] = make_double2(re, im);
     }
     __syncthreads();
     fft_slab<LOGS, WAVE>(buf, logPc, T, tw);
@@ -198,8 +208,8 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
         const uint32_t r = r0 + rr;
         if (r >= H) continue;
         const int t = rr >> 1;
-        const double2 zk = buf[pad((t << logPc) + k)];
-        const double2 zn = cconj(buf[pad((t << logPc) + ((Pc - k) & (Pc - 1)))]);
+        const double2 zk = buf[at(t, k, logPc)];
+        const double2 zn = cconj(buf[at(t, (Pc - k) & (Pc - 1), logPc)]);
         double2 v;
         if ((rr & 1) == 0) {
             v = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y + zn.y));
@@ -211,16 +221,20 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
     }
 }
 
-// cols: block (item, group of C columns).  mode 0: item = left matrix l, writes its full
-// column spectra Lhat[l][k][rho].  mode 1: item = pair q0 + item, writes
-// Z[item][k][rho] = Pr * IFFT(Lhat[l] * conj(FFT(S[r])))  along rho.
+// cols: block (item, group of C columns k).  mode 0: item = left matrix l, writes its full
+// column spectra Lhat[l][k][rho] (the FFT along rho of SL[l][k][0..H), zero padded).
+// mode 1: item = pair q0 + item (left l, right r): X = the same FFT of SR[r][k], then a
+// second FFT of conj(Lhat) * X, conjugated: Z[item][k][rho] = Pr * IFFT_rho(Lhat * conj(X)).
+// Blocks are ordered column-group major through the XCD remap, so the pairs that read the
+// same Lhat columns run on one XCD and find them in its L2.
 template <int LOGS, bool WAVE>
 __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S, double2* __restrict__ Lhat,
                                                    double2* __restrict__ Z, int mode, int family, uint32_t n,
-                                                   uint32_t m, uint32_t q0, uint32_t H, uint32_t Kc, int logPr,
-                                                   int logC, uint32_t cgroups, const double2* __restrict__ tw) {
+                                                   uint32_t m, uint32_t q0, uint32_t items, uint32_t H, uint32_t Kc,
+                                                   int logPr, int logC, const double2* __restrict__ tw) {
     extern __shared__ double2 buf[];
-    const uint32_t item = blockIdx.x / cgroups, cg = blockIdx.x - item * cgroups;
+    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t cg = bid / items, item = bid - cg * items;
     const int Pr = 1 << logPr, C = 1 << logC;
     const uint32_t k0 = cg << logC;
     uint32_t l = item, r = item;
@@ -228,7 +242,7 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
     const double2* Sm = S + (size_t)r * Kc * H;
     for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
         const uint32_t k = k0 + (idx >> logPr), rho = idx & (Pr - 1);
-        buf[pad(idx)] = (k < Kc && rho < H) ? Sm[(size_t)k * H + rho] : make_double2(0.0, 0.0);
+        buf[at(idx >> logPr, rho, logPr)] = (k < Kc && rho < H) ? Sm[(size_t)k * H + rho] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_slab<LOGS, WAVE>(buf, logPr, C, tw);
@@ -236,7 +250,7 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
     if (mode == 0) {
         for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
             const uint32_t k = k0 + (idx >> logPr);
-            if (k < Kc) Ll[(size_t)k * Pr + (idx & (Pr - 1))] = buf[pad(idx)];
+            if (k < Kc) Ll[(size_t)k * Pr + (idx & (Pr - 1))] = buf[at(idx >> logPr, idx & (Pr - 1), logPr)];
         }
         return;
     }
@@ -244,19 +258,30 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
     for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
         const uint32_t k = k0 + (idx >> logPr);
         const double2 lh = k < Kc ? Ll[(size_t)k * Pr + (idx & (Pr - 1))] : make_double2(0.0, 0.0);
-        buf[pad(idx)] = cmul(cconj(lh), buf[pad(idx)]);
+        double2& v = buf[at(idx >> logPr, idx & (Pr - 1), logPr)];
+        v = cmul(cconj(lh), v);
     }
     __syncthreads();
     fft_slab<LOGS, WAVE>(buf, logPr, C, tw);
     double2* Zq = Z + (size_t)item * Kc * Pr;
     for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
         const uint32_t k = k0 + (idx >> logPr);
-        if (k < Kc) Zq[(size_t)k * Pr + (idx & (Pr - 1))] = cconj(buf[pad(idx)]);
+        if (k < Kc) Zq[(size_t)k * Pr + (idx & (Pr - 1))] = cconj(buf[at(idx >> logPr, idx & (Pr - 1), logPr)]);
     }
 }
 
-// rows_inv: block (pair item, group of 2T output rows oy).  Output row oy = dy + H - 1 reads
-// Z row rho = dy mod Pr; output column ox = dx + W - 1 takes circular column dx mod Pc.
+// Output row of circular row rho: dy = rho for rho < H, rho - Pr for rho > Pr - H; -1 for the
+// rows in between (not part of the (2H-1)-row result).
+__device__ __forceinline__ int out_row(uint32_t rho, uint32_t H, uint32_t Pr) {
+    if (rho < H) return (int)(rho + H - 1);
+    if (rho > Pr - H) return (int)(rho - (Pr - H + 1));
+    return -1;
+}
+
+// rows_inv: block (pair item, group of 2T consecutive circular rows rho0 = 2T*g ...), so that
+// each thread's two rows are adjacent in Z and a group's rows are one aligned run.  Row rho
+// goes to output row out_row(rho); output column ox = dx + W - 1 takes circular column
+// dx mod Pc.
 template <typename Tout, int LOGS, bool WAVE>
 __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict__ Z, Tout* __restrict__ O,
                                                        uint32_t q0, uint32_t H, uint32_t W, int logPc, int logPr,
@@ -264,50 +289,41 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
                                                        const double2* __restrict__ tw) {
     extern __shared__ double2 buf[];
     const uint32_t item = blockIdx.x / groups, g = blockIdx.x - item * groups;
-    const int Pc = 1 << logPc, Pr = 1 << logPr, T = 1 << logT;
+    const int Pc = 1 << logPc, T = 1 << logT;
+    const uint32_t Pr = 1u << logPr;
     const int half = Pc / 2;
     const uint32_t Kc = (uint32_t)half + 1;
-    const uint32_t SY = 2 * H - 1, SX = 2 * W - 1;
-    const uint32_t oy0 = g << (logT + 1);
-    const double2* Zq = Z + (size_t)item * Kc * Pr;
-    // Two phases: even rows (A) set every slot, odd rows (B) add theirs.  The buffer holds
-    // conj(Y), Y[k] = A[k] + i B[k] extended by A[Pc-k] = conj(A[k]).
-    for (int phase = 0; phase < 2; ++phase) {
-        for (int idx = threadIdx.x; idx < (int)Kc << logT; idx += kFT) {
-            const int kk = idx >> logT, t = idx & (T - 1);
-            const uint32_t oy = oy0 + 2 * t + phase;
-            double2 v = make_double2(0.0, 0.0);
-            if (oy < SY) {
-                const uint32_t rho = (oy - (H - 1)) & (uint32_t)(Pr - 1);
-                v = Zq[(size_t)kk * Pr + rho];
-            }
-            const int base = t << logPc;
-            if (phase == 0) {
-                buf[pad(base + kk)] = cconj(v);
-                if (kk > 0 && kk < half) buf[pad(base + Pc - kk)] = v;
-            } else {
-                const double2 a = mul_mi(cconj(v)), b = mul_mi(v);
-                double2& s0 = buf[pad(base + kk)];
-                s0 = cadd(s0, a);
-                if (kk > 0 && kk < half) {
-                    double2& s1 = buf[pad(base + Pc - kk)];
-                    s1 = cadd(s1, b);
-                }
-            }
-        }
-        __syncthreads();
-    }
-    fft_slab<LOGS, WAVE>(buf, logPc, T, tw);
-    Tout* Oq = O + (size_t)(q0 + item) * SY * SX;
+    const uint32_t SX = 2 * W - 1;
+    const uint32_t rho0 = g << (logT + 1);
     const int rows2 = 2 << logT;
-    for (int idx = threadIdx.x; idx < rows2 * (int)SX; idx += kFT) {
-        const int rr = idx / (int)SX, ox = idx - rr * (int)SX;
-        const uint32_t oy = oy0 + rr;
-        if (oy >= SY) continue;
-        const int c = (ox - (int)(W - 1)) & (Pc - 1);
-        const double2 f = buf[pad(((rr >> 1) << logPc) + c)];
-        const double v = (rr & 1) ? -f.y : f.x;
-        Oq[(size_t)oy * SX + ox] = (Tout)(v * scale);
+    // Groups wholly between the two halves of the result have nothing to write.
+    if (out_row(rho0, H, Pr) < 0 && out_row(rho0 + rows2 - 1, H, Pr) < 0 && rho0 >= H) return;
+    const double2* Zq = Z + (size_t)item * Kc * Pr;
+    // One thread per (column kk, transform t) loads the two rows' values A[kk], B[kk] (32
+    // adjacent bytes) and writes both slots kk and Pc-kk of conj(Y), Y[k] = A[k] + i B[k]
+    // extended by A[Pc-k] = conj(A[k]), B[Pc-k] = conj(B[k]).
+    for (int idx = threadIdx.x; idx < (int)Kc << logT; idx += kFT) {
+        const int kk = idx >> logT, t = idx & (T - 1);
+        const double2* zc = Zq + (size_t)kk * Pr + rho0 + 2 * t;
+        const double2 A = zc[0], B = zc[1];
+        buf[at(t, kk, logPc)] = cadd(cconj(A), mul_mi(cconj(B)));
+        if (kk > 0 && kk < half) buf[at(t, Pc - kk, logPc)] = cadd(A, mul_mi(B));
+    }
+    __syncthreads();
+    fft_slab<LOGS, WAVE>(buf, logPc, T, tw);
+    // IFFT(Y) = conj(FFT(conj(Y))) / Pc: row A is the real part, row B minus the imaginary.
+    Tout* Oq = O + (size_t)(q0 + item) * (2 * H - 1) * SX;
+    const int lane = threadIdx.x & 63;
+    for (int rr = threadIdx.x >> 6; rr < rows2; rr += kFT / 64) {
+        const int oy = out_row(rho0 + rr, H, Pr);
+        if (oy < 0) continue;
+        Tout* orow = Oq + (size_t)oy * SX;
+        const int t = rr >> 1;
+        for (int ox = lane; ox < (int)SX; ox += 64) {
+            const int c = (ox - (int)(W - 1)) & (Pc - 1);
+            const double2 f = buf[at(t, c, logPc)];
+            orow[ox] = (Tout)(((rr & 1) ? -f.y : f.x) * scale);
+        }
     }
 }
 
@@ -364,20 +380,25 @@ struct slab_shape {
     uint32_t lds;  // dynamic LDS bytes
 };
 
-slab_shape shape_for(int logP, uint32_t want) {
+// Smallest slab (2^11 values, or larger when min_T transforms of 2^logP need it) and the
+// transforms per block: as many as `want` asks for, within the slab.  Transforms of up to
+// 512 points are wave-owned (T >= 4 then fits the 2048-value slab).
+slab_shape shape_for(int logP, uint32_t want, int log_min_T) {
     slab_shape v{};
-    v.logS = logP <= 11 ? 11 : 12;
-    v.wave = logP <= 9;  // T >= 4 then fits the 2048-value slab
-    int lt = v.wave ? 2 : 0;
+    v.logS = std::max(11, logP + log_min_T);
+    v.wave = logP <= 9;
+    int lt = v.wave ? 2 : log_min_T;
     while (lt + logP < v.logS && (1u << lt) < want) ++lt;
     v.logT = lt;
-    v.lds = (uint32_t)(((1u << (logP + lt)) * 9 / 8) * sizeof(double2));
+    v.lds = (uint32_t)(((1u << lt) * ((1u << logP) + (1u << logP) / 8 + 1)) * sizeof(double2));
     return v;
 }
 
 template <typename F>
 void with_shape(const slab_shape& v, F&& f) {
-    if (v.logS == 12)
+    if (v.logS == 13)
+        f(std::integral_constant<int, 13>{}, std::false_type{});
+    else if (v.logS == 12)
         f(std::integral_constant<int, 12>{}, std::false_type{});
     else if (v.wave)
         f(std::integral_constant<int, 11>{}, std::true_type{});
@@ -404,15 +425,15 @@ void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, hipStrea
     const uint32_t pairs = a.pairs;
     const double2* tw = twiddles();
 
-    // Slab shape per kernel: transforms per block, and the variant (slab of 2048 values with
-    // wave-owned transforms for P <= 512; 2048 block-wide for P = 1024, 2048; 4096 for 4096).
-    const slab_shape v1 = shape_for(logPc, (H + 1) / 2);  // rows_fwd: 2T input rows per block
-    const slab_shape vc = shape_for(logPr, Kc);           // cols: C columns per block
-    const slab_shape v3 = shape_for(logPc, H);            // rows_inv: 2T output rows per block
+    // Slab shape per kernel (see shape_for): rows_fwd 2T input rows per block, cols C columns,
+    // rows_inv 2T output rows.
+    const slab_shape v1 = shape_for(logPc, (H + 1) / 2, 0);
+    const slab_shape vc = shape_for(logPr, Kc, 0);
+    const slab_shape v3 = shape_for(logPc, Pr / 2, 0);  // 2T <= Pr
     const int logT1 = v1.logT, logC = vc.logT, logT3 = v3.logT;
     const uint32_t g1 = (H + (2u << logT1) - 1) >> (logT1 + 1);
     const uint32_t cg = (Kc + (1u << logC) - 1) >> logC;
-    const uint32_t g3 = (2 * H - 1 + (2u << logT3) - 1) >> (logT3 + 1);
+    const uint32_t g3 = Pr >> (logT3 + 1);
 
     const size_t s_bytes = sizeof(double2) * (size_t)Kc * H;
     const size_t col_bytes = sizeof(double2) * (size_t)Kc * Pr;
@@ -423,41 +444,35 @@ void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, hipStrea
     const uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(pairs, (size_t(2) << 30) / col_bytes));
     double2* Zw = static_cast<double2*>(workspace(WS_FK_Z, col_bytes * chunk, s));
 
-    auto rows_fwd = [&](const Tio* X, uint32_t nmat, double2* S) {
+    {
+        timed t0("fftk_rows_fwd", s);
         with_shape(v1, [&](auto LS, auto WV) {
             auto k = rows_fwd_kernel<Tio, decltype(LS)::value, decltype(WV)::value>;
             allow_lds(k, v1.lds);
-            hipLaunchKernelGGL(k, dim3(nmat * g1), dim3(kFT), v1.lds, s, X, H, W, logPc, logT1, g1, S, tw);
+            hipLaunchKernelGGL(k, dim3((nl + nr) * g1), dim3(kFT), v1.lds, s, L, R, nl, H, W, logPc, logT1, g1, SL, SR,
+                               tw);
         });
-    };
+    }
     auto cols = [&](const double2* S, double2* Zout, int mode, uint32_t items, uint32_t q0) {
         with_shape(vc, [&](auto LS, auto WV) {
             auto k = cols_kernel<decltype(LS)::value, decltype(WV)::value>;
             allow_lds(k, vc.lds);
             hipLaunchKernelGGL(k, dim3(items * cg), dim3(kFT), vc.lds, s, S, LH, Zout, mode, a.family, a.n, a.m, q0,
-                               H, Kc, logPr, logC, cg, tw);
+                               items, H, Kc, logPr, logC, tw);
         });
     };
     {
-        timed t0("fftk_rows_fwd_left", s);
-        rows_fwd(L, nl, SL);
-    }
-    {
         timed t1("fftk_cols_left", s);
         cols(SL, nullptr, 0, nl, 0);
-    }
-    {
-        timed t2("fftk_rows_fwd", s);
-        rows_fwd(R, nr, SR);
     }
     const double scale = 1.0 / ((double)Pr * (double)Pc);
     for (uint32_t q0 = 0; q0 < pairs; q0 += chunk) {
         const uint32_t nq = std::min(chunk, pairs - q0);
         {
-            timed t3("fftk_cols", s);
+            timed t2("fftk_cols", s);
             cols(SR, Zw, 1, nq, q0);
         }
-        timed t4("fftk_rows_inv", s);
+        timed t3("fftk_rows_inv", s);
         with_shape(v3, [&](auto LS, auto WV) {
             auto k = rows_inv_kernel<Tio, decltype(LS)::value, decltype(WV)::value>;
             allow_lds(k, v3.lds);

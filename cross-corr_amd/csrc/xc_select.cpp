@@ -25,7 +25,7 @@ namespace xc {
 
 namespace {
 
-constexpr uint32_t kFeatures = 13;
+constexpr uint32_t kFeatures = 14;
 
 // ------------------------------------------------------------------ candidates
 std::string candidate_name(uint32_t i) {
@@ -112,8 +112,10 @@ void check_candidates(const model& m) {
 
 // Host inference (double precision) of the model's MLP; returns predicted ms per candidate.
 void predict(const model& m, const float* feat, float* ms) {
+    // Features are clamped to the training range first: the network is not trusted outside it.
     std::vector<double> a(feat, feat + kFeatures), b;
-    for (uint32_t i = 0; i < kFeatures; ++i) a[i] = (a[i] - m.f_mean[i]) / m.f_std[i];
+    for (uint32_t i = 0; i < kFeatures; ++i)
+        a[i] = (std::min(std::max(a[i], m.f_min[i]), m.f_max[i]) - m.f_mean[i]) / m.f_std[i];
     size_t off = 0;
     for (uint32_t l = 0; l < m.shape.layers; ++l) {
         const uint32_t din = m.shape.dims[l], dout = m.shape.dims[l + 1];
@@ -233,8 +235,10 @@ xc_options candidate(uint32_t i) {
     return o;
 }
 
-// Features: log2 of rows, cols, n, m, results and MACs, log2 of the FFT points transformed,
-// power-of-two flags of the padded FFT sizes, and the family one-hot.
+// Features: log2 of rows, cols, n, m, results and MACs; log2 of the FFT points transformed
+// at the reference's padding (2H x 2W); log2 of the power-of-two padding's area over the
+// needed (2H-1)(2W-1); log2 of the largest prime factor of 2H and of 2W (how well the
+// reference padding factors); the family one-hot.
 uint32_t workload_features(const xc_workload& w, float* f, uint32_t cap) {
     const int64_t k = num_results((int)w.family, w.n, w.m);
     if (w.family > XC_N_TO_M || k < 0 || w.rows == 0 || w.cols == 0)
@@ -249,7 +253,21 @@ uint32_t workload_features(const xc_workload& w, float* f, uint32_t cap) {
     case XC_N_TO_MN: transforms = (double)w.n + 2.0 * w.m; break;
     default: transforms = (double)w.n + w.m + (double)k; break;
     }
-    auto pow2 = [](uint32_t x) { return (x & (x - 1)) == 0 ? 1.f : 0.f; };
+    auto lpf = [](uint32_t x) {
+        uint32_t best = 1;
+        for (uint32_t p = 2; p * p <= x; ++p)
+            while (x % p == 0) {
+                best = p;
+                x /= p;
+            }
+        return x > 1 ? std::max(best, x) : best;
+    };
+    auto pow2ceil = [](uint32_t x) {
+        uint32_t p = 1;
+        while (p < x) p <<= 1;
+        return p;
+    };
+    const double pr = std::max(16u, pow2ceil(2 * w.rows - 1)), pc = std::max(16u, pow2ceil(2 * w.cols - 1));
     f[0] = (float)lr;
     f[1] = (float)lc;
     f[2] = (float)std::log2((double)w.n);
@@ -257,9 +275,10 @@ uint32_t workload_features(const xc_workload& w, float* f, uint32_t cap) {
     f[4] = (float)lk;
     f[5] = (float)(lk + 2 * lr + 2 * lc);
     f[6] = (float)(std::log2(transforms) + 2 + lr + lc);
-    f[7] = pow2(2 * w.rows);
-    f[8] = pow2(2 * w.cols);
-    for (uint32_t i = 0; i < 4; ++i) f[9 + i] = w.family == i ? 1.f : 0.f;
+    f[7] = (float)std::log2(pr * pc / ((2.0 * w.rows - 1) * (2.0 * w.cols - 1)));
+    f[8] = (float)std::log2((double)lpf(2 * w.rows));
+    f[9] = (float)std::log2((double)lpf(2 * w.cols));
+    for (uint32_t i = 0; i < 4; ++i) f[10 + i] = w.family == i ? 1.f : 0.f;
     return kFeatures;
 }
 

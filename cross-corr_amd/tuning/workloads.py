@@ -1,7 +1,8 @@
 """Seeded random workloads for the cost model's timing table.
 
-A workload is (family, rows, cols, n, m).  Sizes are log-uniform in [16, 1024] (square
-three times in four), batch counts log-uniform per family, and the total work is capped
+A workload is (family, rows, cols, n, m).  Sizes are powers of two from 16 to 1024 for one
+workload in three, else log-uniform in [16, 1024] (square three times in four); batch counts
+are log-uniform per family, and the total work is capped
 at 2e12 multiply-adds (and 512 results) so that every candidate method, the FP32 direct
 kernel included, can be timed in well under a second.
 """
@@ -26,7 +27,9 @@ def random_workloads(count, seed):
     out = []
     for _ in range(count):
         fam = FAMILIES[rng.randrange(4)]
-        if rng.random() < 0.75:
+        if rng.random() < 1 / 3:
+            rows = cols = 2 ** rng.randint(4, 10)
+        elif rng.random() < 0.75:
             rows = cols = _logu(rng, 16, 1024)
         else:
             rows, cols = _logu(rng, 16, 1024), _logu(rng, 16, 1024)

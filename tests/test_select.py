@@ -21,9 +21,22 @@ def py_features(family, rows, cols, n, m):
     k = {0: 1, 1: m, 2: m, 3: n * m}[family]
     transforms = {0: 3, 1: 1 + 2 * m, 2: n + 2 * m, 3: n + m + k}[family]
     lr, lc = np.log2(rows), np.log2(cols)
-    pow2 = lambda v: 1.0 if v & (v - 1) == 0 else 0.0
+
+    def lpf(v):  # largest prime factor
+        best, p = 1, 2
+        while p * p <= v:
+            while v % p == 0:
+                best, v = p, v // p
+            p += 1
+        return max(best, v) if v > 1 else best
+
+    def p2(v):
+        return max(16, 1 << int(np.ceil(np.log2(v))))
+
+    pad = np.log2(p2(2 * rows - 1) * p2(2 * cols - 1) / ((2 * rows - 1) * (2 * cols - 1)))
     return [lr, lc, np.log2(n), np.log2(m), np.log2(k), np.log2(k) + 2 * lr + 2 * lc,
-            np.log2(transforms) + 2 + lr + lc, pow2(2 * rows), pow2(2 * cols)] + [float(family == i) for i in range(4)]
+            np.log2(transforms) + 2 + lr + lc, pad, np.log2(lpf(2 * rows)), np.log2(lpf(2 * cols))] + \
+        [float(family == i) for i in range(4)]
 
 
 def parse_model(path):
@@ -58,7 +71,8 @@ def x():
 
 
 def test_features_match_definition(x):
-    for fam, r, c, n, m in [(0, 16, 16, 1, 1), (1, 256, 256, 1, 64), (2, 100, 37, 3, 12), (3, 64, 64, 16, 64)]:
+    for fam, r, c, n, m in [(0, 16, 16, 1, 1), (1, 256, 256, 1, 64), (2, 100, 37, 3, 12), (3, 64, 64, 16, 64),
+                            (0, 1, 1, 1, 1), (0, 177, 1024, 1, 1)]:
         np.testing.assert_allclose(x.features(FAM[fam], r, c, n, m), py_features(fam, r, c, n, m), rtol=1e-6)
 
 
@@ -67,7 +81,8 @@ def test_host_inference_matches_numpy_forward(x):
     dims = [int(v) for v in mdl["dims"]]
     assert mdl["candidates"] == [c[0] for c in x.candidates()]
     for fam, r, c, n, m in [(1, 256, 256, 1, 64), (3, 64, 64, 16, 64), (0, 40, 900, 1, 1)]:
-        f = (np.array(py_features(fam, r, c, n, m)) - mdl["feat_mean"]) / mdl["feat_std"]
+        f = np.clip(np.array(py_features(fam, r, c, n, m)), mdl["feat_min"], mdl["feat_max"])
+        f = (f - mdl["feat_mean"]) / mdl["feat_std"]
         y, _ = mlp_ref.forward(dims, mdl["params"], f[None, :])
         want = 2.0 ** (y[0] * mdl["out_std"] + mdl["out_mean"])
         np.testing.assert_allclose(x.predict(FAM[fam], r, c, n, m), want, rtol=1e-5)
