@@ -110,7 +110,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--methods", default="fp32,bf16x3,fft64,fft64k,auto")
     ap.add_argument("--cpu-pairs", type=int, default=3, help="pairs timed for the CPU baseline sample")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB device copy 3 times (the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings into bytes), then exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.rand(64 << 20, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this workload runs on one GPU; --gpus {args.gpus} is not supported "
               "(pairs are independent; the sharding design is described in DESIGN.md)", file=sys.stderr)
