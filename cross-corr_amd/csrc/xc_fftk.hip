@@ -37,12 +37,23 @@ constexpr int kFT = 256;              // threads per block
 constexpr int kLogMaxP = 12;          // longest transform: 4096 (the twiddle table's length)
 constexpr int kSlab = 1 << kLogMaxP;  // twiddle table entries
 
-// LDS slot of element j of transform t (length 2^logP): one pad slot per 8 values, and a
+// LDS slot of element j of transform t (length P = 2^LOGP): one pad slot per 8 values, and a
 // transform stride of 9P/8 + 1 slots, so that lanes touching the same j of consecutive
-// transforms, or j = 8i, hit different banks.
-__device__ __forceinline__ int at(int t, int j, int logP) {
-    return t * ((1 << logP) + (1 << logP) / 8 + 1) + j + (j >> 3);
+// transforms, or j = 8i, hit different banks.  With P a compile-time constant the pad and
+// stride arithmetic folds into immediate LDS offsets.
+template <int LOGP>
+__device__ __forceinline__ int at(int t, int j) {
+    constexpr int S = (1 << LOGP) + (1 << LOGP) / 8 + 1;
+    return t * S + j + (j >> 3);
 }
+
+// Slab (values per block) and wave ownership of a transform length: 2^11 values, or one
+// transform of 4096; transforms of up to 512 points are owned by single waves (T >= 4).
+template <int LOGP>
+struct slab_of {
+    static constexpr int LOGS = LOGP <= 11 ? 11 : LOGP;
+    static constexpr bool WAVE = LOGP <= 9;
+};
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -105,18 +116,17 @@ __device__ __forceinline__ void slab_sync() {
     }
 }
 
-// One Stockham radix-R pass over T transforms of length P = 2^logP in buf (slots `at`),
-// T * P <= 2^LOGS.  logp: log2 of the product of the radices already
-// applied.  Butterfly i of a transform reads elements i + r*P/R and writes (i - k)*R + k + r*p,
-// k = i mod p, after multiplying input r by exp(-2 pi i r k / (p R)).  Butterflies go to
-// threads block-strided, or (WAVE, T a multiple of 4) as one contiguous range of whole
-// transforms per wave.
-template <int R, int LOGR, int LOGS, bool WAVE>
-__device__ __forceinline__ void fft_pass(double2* buf, int logP, int T, int logp, const double2* __restrict__ tw) {
+// One Stockham radix-R pass over T transforms of length P = 2^LOGP in buf (slots `at`),
+// T * P <= 2^LOGS; 2^LOGPP is the product of the radices already applied.  Butterfly i of a
+// transform reads elements i + r*P/R and writes (i - k)*R + k + r*p, k = i mod p, after
+// multiplying input r by exp(-2 pi i r k / (p R)).  Butterflies go to threads block-strided,
+// or (WAVE, T a multiple of 4) as one contiguous range of whole transforms per wave.
+template <int R, int LOGR, int LOGP, int LOGPP, int LOGS, bool WAVE>
+__device__ __forceinline__ void fft_pass(double2* buf, int T, const double2* __restrict__ tw) {
     constexpr int MAXB = (1 << LOGS) / R / kFT;
     static_assert(MAXB >= 1, "slab too small for the radix");
-    const int logPR = logP - LOGR;
-    const int nb = T << logPR;
+    constexpr int LOGPR = LOGP - LOGR;
+    const int nb = T << LOGPR;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int per_wave = nb >> 2;
     double2 u[MAXB][R];
@@ -125,9 +135,9 @@ __device__ __forceinline__ void fft_pass(double2* buf, int logP, int T, int logp
         const bool ok = WAVE ? lane + 64 * q < per_wave : (int)threadIdx.x + kFT * q < nb;
         const int b = WAVE ? wave * per_wave + lane + 64 * q : (int)threadIdx.x + kFT * q;
         if (ok) {
-            const int t = b >> logPR, i = b & ((1 << logPR) - 1);
+            const int t = b >> LOGPR, i = b & ((1 << LOGPR) - 1);
 #pragma unroll
-            for (int r = 0; r < R; ++r) u[q][r] = buf[at(t, i + (r << logPR), logP)];
+            for (int r = 0; r < R; ++r) u[q][r] = buf[at<LOGP>(t, i + (r << LOGPR))];
         }
     }
     slab_sync<WAVE>();
@@ -136,10 +146,10 @@ __device__ __forceinline__ void fft_pass(double2* buf, int logP, int T, int logp
         const bool ok = WAVE ? lane + 64 * q < per_wave : (int)threadIdx.x + kFT * q < nb;
         const int b = WAVE ? wave * per_wave + lane + 64 * q : (int)threadIdx.x + kFT * q;
         if (ok) {
-            const int t = b >> logPR, i = b & ((1 << logPR) - 1);
-            const int k = i & ((1 << logp) - 1);
-            if (logp > 0) {
-                const double2 w1 = tw[k << (kLogMaxP - logp - LOGR)];
+            const int t = b >> LOGPR, i = b & ((1 << LOGPR) - 1);
+            const int k = i & ((1 << LOGPP) - 1);
+            if constexpr (LOGPP > 0) {
+                const double2 w1 = tw[k << (kLogMaxP - LOGPP - LOGR)];
                 double2 w = w1;
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
@@ -150,39 +160,47 @@ __device__ __forceinline__ void fft_pass(double2* buf, int logP, int T, int logp
             dft<R>(u[q]);
             const int j = ((i - k) << LOGR) + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[at(t, j + (r << logp), logP)] = u[q][r];
+            for (int r = 0; r < R; ++r) buf[at<LOGP>(t, j + (r << LOGPP))] = u[q][r];
         }
     }
     slab_sync<WAVE>();
 }
 
-// Forward FFT of T transforms of length 2^logP in buf; natural order.  The caller has
-// synchronised the block before; the block is synchronised on return.
-template <int LOGS, bool WAVE>
-__device__ void fft_slab(double2* buf, int logP, int T, const double2* __restrict__ tw) {
-    int logp = 0;
-    const int rem = logP % 3;
-    if (rem == 1) {
-        fft_pass<2, 1, LOGS, WAVE>(buf, logP, T, logp, tw);
-        logp += 1;
-    } else if (rem == 2) {
-        fft_pass<4, 2, LOGS, WAVE>(buf, logP, T, logp, tw);
-        logp += 2;
+template <int LOGP, int LOGPP>
+__device__ __forceinline__ void radix8_passes(double2* buf, int T, const double2* __restrict__ tw) {
+    if constexpr (LOGPP < LOGP) {
+        fft_pass<8, 3, LOGP, LOGPP, slab_of<LOGP>::LOGS, slab_of<LOGP>::WAVE>(buf, T, tw);
+        radix8_passes<LOGP, LOGPP + 3>(buf, T, tw);
     }
-    while (logp < logP) {
-        fft_pass<8, 3, LOGS, WAVE>(buf, logP, T, logp, tw);
-        logp += 3;
+}
+
+// Forward FFT of T transforms of length 2^LOGP in buf; natural order.  The caller has
+// synchronised the block before; the block is synchronised on return.
+template <int LOGP>
+__device__ __forceinline__ void fft_slab(double2* buf, int T, const double2* __restrict__ tw) {
+    constexpr int LOGS = slab_of<LOGP>::LOGS;
+    constexpr bool WAVE = slab_of<LOGP>::WAVE;
+    constexpr int rem = LOGP % 3;
+    if constexpr (rem == 1) {
+        fft_pass<2, 1, LOGP, 0, LOGS, WAVE>(buf, T, tw);
+        radix8_passes<LOGP, 1>(buf, T, tw);
+    } else if constexpr (rem == 2) {
+        fft_pass<4, 2, LOGP, 0, LOGS, WAVE>(buf, T, tw);
+        radix8_passes<LOGP, 2>(buf, T, tw);
+    } else {
+        radix8_passes<LOGP, 0>(buf, T, tw);
     }
     if constexpr (WAVE) __syncthreads();
 }
 
 // rows_fwd: block (matrix, group of 2T rows); matrices 0..nl-1 are the left ones (into SL),
 // the rest the right ones (into SR).
-template <typename Tin, int LOGS, bool WAVE>
+template <typename Tin, int LOGPC>
 __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ XL, const Tin* __restrict__ XR,
-                                                       uint32_t nl, uint32_t H, uint32_t W, int logPc, int logT,
-                                                       uint32_t groups, double2* __restrict__ SL,
-                                                       double2* __restrict__ SR, const double2* __restrict__ tw) {
+                                                       uint32_t nl, uint32_t H, uint32_t W, int logT, uint32_t groups,
+                                                       double2* __restrict__ SL, double2* __restrict__ SR,
+                                                       const double2* __restrict__ tw) {
+    constexpr int logPc = LOGPC;
     extern __shared__ double2 buf[];
     const uint32_t matg = blockIdx.x / groups, g = blockIdx.x - matg * groups;
     const bool left = matg < nl;
@@ -196,10 +214,10 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
         const uint32_t a = r0 + 2 * t, b = a + 1;
         const double re = (a < H && (uint32_t)c < W) ? (double)Xm[(size_t)a * W + c] : 0.0;
         const double im = (b < H && (uint32_t)c < W) ? (double)Xm[(size_t)b * W + c] : 0.0;
-        buf[at(t, c, logPc)] = make_double2(re, im);
+        buf[at<LOGPC>(t, c)] = make_double2(re, im);
     }
     __syncthreads();
-    fft_slab<LOGS, WAVE>(buf, logPc, T, tw);
+    fft_slab<LOGPC>(buf, T, tw);
     const uint32_t Kc = (uint32_t)(Pc / 2 + 1);
     const int rows2 = 2 << logT;
     double2* Sm = S + (size_t)mat * Kc * H;
@@ -208,8 +226,8 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
         const uint32_t r = r0 + rr;
         if (r >= H) continue;
         const int t = rr >> 1;
-        const double2 zk = buf[at(t, k, logPc)];
-        const double2 zn = cconj(buf[at(t, (Pc - k) & (Pc - 1), logPc)]);
+        const double2 zk = buf[at<LOGPC>(t, k)];
+        const double2 zn = cconj(buf[at<LOGPC>(t, (Pc - k) & (Pc - 1))]);
         double2 v;
         if ((rr & 1) == 0) {
             v = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y + zn.y));
@@ -227,11 +245,12 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
 // second FFT of conj(Lhat) * X, conjugated: Z[item][k][rho] = Pr * IFFT_rho(Lhat * conj(X)).
 // Blocks are ordered column-group major through the XCD remap, so the pairs that read the
 // same Lhat columns run on one XCD and find them in its L2.
-template <int LOGS, bool WAVE>
+template <int LOGPR>
 __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S, double2* __restrict__ Lhat,
                                                    double2* __restrict__ Z, int mode, int family, uint32_t n,
                                                    uint32_t m, uint32_t q0, uint32_t items, uint32_t H, uint32_t Kc,
-                                                   int logPr, int logC, const double2* __restrict__ tw) {
+                                                   int logC, const double2* __restrict__ tw) {
+    constexpr int logPr = LOGPR;
     extern __shared__ double2 buf[];
     const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t cg = bid / items, item = bid - cg * items;
@@ -242,15 +261,15 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
     const double2* Sm = S + (size_t)r * Kc * H;
     for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
         const uint32_t k = k0 + (idx >> logPr), rho = idx & (Pr - 1);
-        buf[at(idx >> logPr, rho, logPr)] = (k < Kc && rho < H) ? Sm[(size_t)k * H + rho] : make_double2(0.0, 0.0);
+        buf[at<LOGPR>(idx >> logPr, rho)] = (k < Kc && rho < H) ? Sm[(size_t)k * H + rho] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_slab<LOGS, WAVE>(buf, logPr, C, tw);
+    fft_slab<LOGPR>(buf, C, tw);
     double2* Ll = Lhat + (size_t)l * Kc * Pr;
     if (mode == 0) {
         for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
             const uint32_t k = k0 + (idx >> logPr);
-            if (k < Kc) Ll[(size_t)k * Pr + (idx & (Pr - 1))] = buf[at(idx >> logPr, idx & (Pr - 1), logPr)];
+            if (k < Kc) Ll[(size_t)k * Pr + (idx & (Pr - 1))] = buf[at<LOGPR>(idx >> logPr, idx & (Pr - 1))];
         }
         return;
     }
@@ -258,15 +277,15 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
     for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
         const uint32_t k = k0 + (idx >> logPr);
         const double2 lh = k < Kc ? Ll[(size_t)k * Pr + (idx & (Pr - 1))] : make_double2(0.0, 0.0);
-        double2& v = buf[at(idx >> logPr, idx & (Pr - 1), logPr)];
+        double2& v = buf[at<LOGPR>(idx >> logPr, idx & (Pr - 1))];
         v = cmul(cconj(lh), v);
     }
     __syncthreads();
-    fft_slab<LOGS, WAVE>(buf, logPr, C, tw);
+    fft_slab<LOGPR>(buf, C, tw);
     double2* Zq = Z + (size_t)item * Kc * Pr;
     for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
         const uint32_t k = k0 + (idx >> logPr);
-        if (k < Kc) Zq[(size_t)k * Pr + (idx & (Pr - 1))] = cconj(buf[at(idx >> logPr, idx & (Pr - 1), logPr)]);
+        if (k < Kc) Zq[(size_t)k * Pr + (idx & (Pr - 1))] = cconj(buf[at<LOGPR>(idx >> logPr, idx & (Pr - 1))]);
     }
 }
 
@@ -282,11 +301,11 @@ __device__ __forceinline__ int out_row(uint32_t rho, uint32_t H, uint32_t Pr) {
 // each thread's two rows are adjacent in Z and a group's rows are one aligned run.  Row rho
 // goes to output row out_row(rho); output column ox = dx + W - 1 takes circular column
 // dx mod Pc.
-template <typename Tout, int LOGS, bool WAVE>
+template <typename Tout, int LOGPC>
 __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict__ Z, Tout* __restrict__ O,
-                                                       uint32_t q0, uint32_t H, uint32_t W, int logPc, int logPr,
-                                                       int logT, uint32_t groups, double scale,
-                                                       const double2* __restrict__ tw) {
+                                                       uint32_t q0, uint32_t H, uint32_t W, int logPr, int logT,
+                                                       uint32_t groups, double scale, const double2* __restrict__ tw) {
+    constexpr int logPc = LOGPC;
     extern __shared__ double2 buf[];
     const uint32_t item = blockIdx.x / groups, g = blockIdx.x - item * groups;
     const int Pc = 1 << logPc, T = 1 << logT;
@@ -306,11 +325,11 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
         const int kk = idx >> logT, t = idx & (T - 1);
         const double2* zc = Zq + (size_t)kk * Pr + rho0 + 2 * t;
         const double2 A = zc[0], B = zc[1];
-        buf[at(t, kk, logPc)] = cadd(cconj(A), mul_mi(cconj(B)));
-        if (kk > 0 && kk < half) buf[at(t, Pc - kk, logPc)] = cadd(A, mul_mi(B));
+        buf[at<LOGPC>(t, kk)] = cadd(cconj(A), mul_mi(cconj(B)));
+        if (kk > 0 && kk < half) buf[at<LOGPC>(t, Pc - kk)] = cadd(A, mul_mi(B));
     }
     __syncthreads();
-    fft_slab<LOGS, WAVE>(buf, logPc, T, tw);
+    fft_slab<LOGPC>(buf, T, tw);
     // IFFT(Y) = conj(FFT(conj(Y))) / Pc: row A is the real part, row B minus the imaginary.
     Tout* Oq = O + (size_t)(q0 + item) * (2 * H - 1) * SX;
     const int lane = threadIdx.x & 63;
@@ -321,7 +340,7 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
         const int t = rr >> 1;
         for (int ox = lane; ox < (int)SX; ox += 64) {
             const int c = (ox - (int)(W - 1)) & (Pc - 1);
-            const double2 f = buf[at(t, c, logPc)];
+            const double2 f = buf[at<LOGPC>(t, c)];
             orow[ox] = (Tout)(((rr & 1) ? -f.y : f.x) * scale);
         }
     }
@@ -380,30 +399,35 @@ struct slab_shape {
     uint32_t lds;  // dynamic LDS bytes
 };
 
-// Smallest slab (2^11 values, or larger when min_T transforms of 2^logP need it) and the
-// transforms per block: as many as `want` asks for, within the slab.  Transforms of up to
-// 512 points are wave-owned (T >= 4 then fits the 2048-value slab).
-slab_shape shape_for(int logP, uint32_t want, int log_min_T) {
+// The slab of slab_of<LOGP> and the transforms per block: as many as `want` asks for, within
+// the slab.  Transforms of up to 512 points are wave-owned (T >= 4 then fits the 2048-value
+// slab).
+slab_shape shape_for(int logP, uint32_t want) {
     slab_shape v{};
-    v.logS = std::max(11, logP + log_min_T);
+    v.logS = std::max(11, logP);  // as slab_of<LOGP>
     v.wave = logP <= 9;
-    int lt = v.wave ? 2 : log_min_T;
+    int lt = v.wave ? 2 : 0;
     while (lt + logP < v.logS && (1u << lt) < want) ++lt;
     v.logT = lt;
     v.lds = (uint32_t)(((1u << lt) * ((1u << logP) + (1u << logP) / 8 + 1)) * sizeof(double2));
     return v;
 }
 
+// Calls f(std::integral_constant<int, LOGP>) for the run-time transform length 2^logP.
 template <typename F>
-void with_shape(const slab_shape& v, F&& f) {
-    if (v.logS == 13)
-        f(std::integral_constant<int, 13>{}, std::false_type{});
-    else if (v.logS == 12)
-        f(std::integral_constant<int, 12>{}, std::false_type{});
-    else if (v.wave)
-        f(std::integral_constant<int, 11>{}, std::true_type{});
-    else
-        f(std::integral_constant<int, 11>{}, std::false_type{});
+void with_logp(int logP, F&& f) {
+    switch (logP) {
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 9: f(std::integral_constant<int, 9>{}); break;
+    case 10: f(std::integral_constant<int, 10>{}); break;
+    case 11: f(std::integral_constant<int, 11>{}); break;
+    case 12: f(std::integral_constant<int, 12>{}); break;
+    default: fail(XC_ERR_UNSUPPORTED, "own-FFT method: transform length out of range");
+    }
 }
 
 template <typename K>
@@ -427,9 +451,9 @@ void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, hipStrea
 
     // Slab shape per kernel (see shape_for): rows_fwd 2T input rows per block, cols C columns,
     // rows_inv 2T output rows.
-    const slab_shape v1 = shape_for(logPc, (H + 1) / 2, 0);
-    const slab_shape vc = shape_for(logPr, Kc, 0);
-    const slab_shape v3 = shape_for(logPc, Pr / 2, 0);  // 2T <= Pr
+    const slab_shape v1 = shape_for(logPc, (H + 1) / 2);
+    const slab_shape vc = shape_for(logPr, Kc);
+    const slab_shape v3 = shape_for(logPc, Pr / 2);  // 2T <= Pr
     const int logT1 = v1.logT, logC = vc.logT, logT3 = v3.logT;
     const uint32_t g1 = (H + (2u << logT1) - 1) >> (logT1 + 1);
     const uint32_t cg = (Kc + (1u << logC) - 1) >> logC;
@@ -446,19 +470,18 @@ void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, hipStrea
 
     {
         timed t0("fftk_rows_fwd", s);
-        with_shape(v1, [&](auto LS, auto WV) {
-            auto k = rows_fwd_kernel<Tio, decltype(LS)::value, decltype(WV)::value>;
+        with_logp(logPc, [&](auto LP) {
+            auto k = rows_fwd_kernel<Tio, decltype(LP)::value>;
             allow_lds(k, v1.lds);
-            hipLaunchKernelGGL(k, dim3((nl + nr) * g1), dim3(kFT), v1.lds, s, L, R, nl, H, W, logPc, logT1, g1, SL, SR,
-                               tw);
+            hipLaunchKernelGGL(k, dim3((nl + nr) * g1), dim3(kFT), v1.lds, s, L, R, nl, H, W, logT1, g1, SL, SR, tw);
         });
     }
     auto cols = [&](const double2* S, double2* Zout, int mode, uint32_t items, uint32_t q0) {
-        with_shape(vc, [&](auto LS, auto WV) {
-            auto k = cols_kernel<decltype(LS)::value, decltype(WV)::value>;
+        with_logp(logPr, [&](auto LP) {
+            auto k = cols_kernel<decltype(LP)::value>;
             allow_lds(k, vc.lds);
             hipLaunchKernelGGL(k, dim3(items * cg), dim3(kFT), vc.lds, s, S, LH, Zout, mode, a.family, a.n, a.m, q0,
-                               items, H, Kc, logPr, logC, tw);
+                               items, H, Kc, logC, tw);
         });
     };
     {
@@ -473,11 +496,11 @@ void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, hipStrea
             cols(SR, Zw, 1, nq, q0);
         }
         timed t3("fftk_rows_inv", s);
-        with_shape(v3, [&](auto LS, auto WV) {
-            auto k = rows_inv_kernel<Tio, decltype(LS)::value, decltype(WV)::value>;
+        with_logp(logPc, [&](auto LP) {
+            auto k = rows_inv_kernel<Tio, decltype(LP)::value>;
             allow_lds(k, v3.lds);
-            hipLaunchKernelGGL(k, dim3(nq * g3), dim3(kFT), v3.lds, s, (const double2*)Zw, O, q0, H, W, logPc, logPr,
-                               logT3, g3, scale, tw);
+            hipLaunchKernelGGL(k, dim3(nq * g3), dim3(kFT), v3.lds, s, (const double2*)Zw, O, q0, H, W, logPr, logT3,
+                               g3, scale, tw);
         });
     }
     XC_HIP(hipGetLastError());
