@@ -103,6 +103,8 @@ SIGNATURES = {
                                    ctypes.POINTER(ctypes.c_uint32)]),
     "xc_autotune_f32": (_st, [_i32, _vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, ctypes.POINTER(ctypes.c_float),
                               ctypes.POINTER(ctypes.c_uint32), _vp]),
+    "xc_time_candidate_f32": (_st, [_i32, _vp, _vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32,
+                                    ctypes.POINTER(ctypes.c_float), _vp]),
     "xc_autotune_clear": (_st, []),
     "xc_cost_model_load": (_st, [ctypes.c_char_p]),
     "xc_cost_model_predict": (_st, [ctypes.POINTER(Workload), ctypes.POINTER(ctypes.c_float)]),
@@ -297,6 +299,17 @@ def autotune(family, left, right, out, reps=3, stream=None):
     check(fn(fam, _ptr(left), _ptr(right), _ptr(out), rows, cols, n, m, reps, ms, ctypes.byref(best),
              _stream(stream)))
     return [ms[i] for i in range(nc)], best.value
+
+
+def time_candidate(family, left, right, out, index, reps=3, stream=None):
+    """ms of one candidate (xc_time_candidate_f32) on torch device tensors."""
+    fam = FAMILIES[family] if isinstance(family, str) else family
+    n, rows, cols = left.shape
+    m = right.shape[0]
+    ms = ctypes.c_float()
+    check(lib().xc_time_candidate_f32(fam, _ptr(left), _ptr(right), _ptr(out), rows, cols, n, m, index, reps,
+                                      ctypes.byref(ms), _stream(stream)))
+    return ms.value
 
 
 def cost_model_load(path=None):

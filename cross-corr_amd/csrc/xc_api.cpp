@@ -318,6 +318,18 @@ xc_status xc_autotune_f32(xc_family family, const float* left, const float* righ
     });
 }
 
+xc_status xc_time_candidate_f32(xc_family family, const float* left, const float* right, float* out, uint32_t rows,
+                                uint32_t cols, uint32_t n, uint32_t m, uint32_t index, uint32_t reps, float* ms,
+                                xc_stream_t stream) {
+    return guarded([&] {
+        check_family(family, rows, cols, n, m, left, right, out);
+        if (index >= xc::num_candidates()) xc::fail(XC_ERR_INVALID_ARGUMENT, "candidate index out of range");
+        if (!ms) xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        xc::family_args a{family, rows, cols, n, m, (uint32_t)xc::num_results(family, n, m)};
+        *ms = xc::time_candidate_f32(a, left, right, out, index, reps, (hipStream_t)stream);
+    });
+}
+
 xc_status xc_autotune_clear(void) {
     return guarded([&] { xc::autotune_clear(); });
 }

@@ -31,6 +31,8 @@ void cost_model_predict(const xc_workload& w, float* ms);
 int select_method(const xc_workload& w, xc_options& chosen);  // xc_select_source
 uint32_t autotune_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t reps, float* ms,
                       hipStream_t s);
+float time_candidate_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t index,
+                         uint32_t reps, hipStream_t s);
 void autotune_clear();
 xc_options auto_options_f32(const family_args& a, const float* L, const float* R, float* O, hipStream_t s);
 

@@ -42,12 +42,15 @@ struct model {
     xc_mlp_shape shape{};
     std::vector<float> params;
     std::vector<double> f_mean, f_std, f_min, f_max, y_mean, y_std;
+    std::vector<double> c_max_macs;  // per candidate: largest log2 MACs with a measured time
 };
 
 std::mutex g_mu;
 model g_model;
 bool g_model_tried = false;
 std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, int>, uint32_t> g_tuned;
+// The model's answer per workload (a prediction costs a few microseconds of host time).
+std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>, uint32_t> g_predicted;
 
 std::string default_model_path() {
     if (const char* e = std::getenv("XC_COST_MODEL")) return e;
@@ -78,8 +81,8 @@ model parse_model(const std::string& path) {
     m.path = path;
     std::string magic;
     int version = 0;
-    if (!(in >> magic >> version) || magic != "xcorr-cost-model" || version != 1)
-        fail(XC_ERR_INVALID_ARGUMENT, "not a cost model file (version 1): " + path);
+    if (!(in >> magic >> version) || magic != "xcorr-cost-model" || version != 2)
+        fail(XC_ERR_INVALID_ARGUMENT, "not a cost model file (version 2): " + path);
     read_list(in, "candidates", m.cands);
     std::vector<uint32_t> dims;
     read_list(in, "dims", dims);
@@ -93,10 +96,12 @@ model parse_model(const std::string& path) {
     read_list(in, "feat_max", m.f_max);
     read_list(in, "out_mean", m.y_mean);
     read_list(in, "out_std", m.y_std);
+    read_list(in, "cand_max_log2_macs", m.c_max_macs);
     read_list(in, "params", m.params);
     const uint32_t nin = m.shape.dims[0], nout = m.shape.dims[m.shape.layers];
     if (nin != kFeatures || m.f_mean.size() != nin || m.f_std.size() != nin || m.f_min.size() != nin ||
         m.f_max.size() != nin || m.cands.size() != nout || m.y_mean.size() != nout || m.y_std.size() != nout ||
+        m.c_max_macs.size() != nout ||
         m.params.size() != mlp_num_params(m.shape))
         fail(XC_ERR_INVALID_ARGUMENT, "cost model file: inconsistent sizes in " + path);
     m.loaded = true;
@@ -288,6 +293,7 @@ void cost_model_load(const char* path) {
     std::lock_guard<std::mutex> lock(g_mu);
     g_model = std::move(m);
     g_model_tried = true;
+    g_predicted.clear();
 }
 
 void cost_model_predict(const xc_workload& w, float* ms) {
@@ -316,16 +322,28 @@ int select_method(const xc_workload& w, xc_options& chosen) {
     if (auto_mode() == "autotune") return XC_SELECT_NONE;
     ensure_model_locked();
     if (!g_model.loaded || !in_domain(g_model, f)) return XC_SELECT_NONE;
+    const auto wkey = std::make_tuple(w.family, w.rows, w.cols, w.n, w.m);
+    auto pit = g_predicted.find(wkey);
+    if (pit != g_predicted.end()) {
+        chosen = candidate(pit->second);
+        return XC_SELECT_MODEL;
+    }
     std::vector<float> ms(g_model.cands.size());
     predict(g_model, f, ms.data());
-    const uint32_t best = (uint32_t)(std::min_element(ms.begin(), ms.end()) - ms.begin());
+    // A candidate is considered only up to twice the largest work it was timed on.
+    uint32_t best = UINT32_MAX;
+    for (uint32_t c = 0; c < ms.size(); ++c)
+        if (f[5] <= g_model.c_max_macs[c] + 1.0 && (best == UINT32_MAX || ms[c] < ms[best])) best = c;
+    if (best == UINT32_MAX) return XC_SELECT_NONE;
+    if (g_predicted.size() > 4096) g_predicted.clear();
+    g_predicted[wkey] = best;
     chosen = candidate(best);
     return XC_SELECT_MODEL;
 }
 
-uint32_t autotune_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t reps, float* ms,
-                      hipStream_t s) {
-    const uint32_t n = num_candidates();
+float time_candidate_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t index,
+                         uint32_t reps, hipStream_t s) {
+    const xc_options o = candidate(index);
     reps = std::max<uint32_t>(reps, 1);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     XC_HIP(hipEventCreate(&e0));
@@ -337,31 +355,37 @@ uint32_t autotune_f32(const family_args& a, const float* L, const float* R, floa
             (void)hipEventDestroy(b);
         }
     } eg{e0, e1};
+    float t = INFINITY;
+    try {
+        // Warm-up (plans, kernel attributes, workspace), timed too: a slow candidate is not
+        // repeated.
+        XC_HIP(hipEventRecord(e0, s));
+        run_family_f32(a, L, R, O, o, s);
+        XC_HIP(hipEventRecord(e1, s));
+        XC_HIP(hipEventSynchronize(e1));
+        float warm = 0.f;
+        XC_HIP(hipEventElapsedTime(&warm, e0, e1));
+        const uint32_t r = warm > 250.f ? 1 : reps;
+        XC_HIP(hipEventRecord(e0, s));
+        for (uint32_t k = 0; k < r; ++k) run_family_f32(a, L, R, O, o, s);
+        XC_HIP(hipEventRecord(e1, s));
+        XC_HIP(hipEventSynchronize(e1));
+        XC_HIP(hipEventElapsedTime(&t, e0, e1));
+        t /= (float)r;
+    } catch (const error& e) {
+        if (e.status == XC_ERR_HIP) throw;  // a device failure is not a slow candidate
+        t = INFINITY;
+    }
+    return t;
+}
+
+uint32_t autotune_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t reps, float* ms,
+                      hipStream_t s) {
+    const uint32_t n = num_candidates();
     uint32_t best = 0;
     float best_ms = INFINITY;
     for (uint32_t i = 0; i < n; ++i) {
-        const xc_options o = candidate(i);
-        float t = INFINITY;
-        try {
-            // Warm-up (plans, kernel attributes, workspace), timed too: a slow candidate
-            // is not repeated.
-            XC_HIP(hipEventRecord(e0, s));
-            run_family_f32(a, L, R, O, o, s);
-            XC_HIP(hipEventRecord(e1, s));
-            XC_HIP(hipEventSynchronize(e1));
-            float warm = 0.f;
-            XC_HIP(hipEventElapsedTime(&warm, e0, e1));
-            const uint32_t r = warm > 250.f ? 1 : reps;
-            XC_HIP(hipEventRecord(e0, s));
-            for (uint32_t k = 0; k < r; ++k) run_family_f32(a, L, R, O, o, s);
-            XC_HIP(hipEventRecord(e1, s));
-            XC_HIP(hipEventSynchronize(e1));
-            XC_HIP(hipEventElapsedTime(&t, e0, e1));
-            t /= (float)r;
-        } catch (const error& e) {
-            if (e.status == XC_ERR_HIP) throw;  // a device failure is not a slow candidate
-            t = INFINITY;
-        }
+        const float t = time_candidate_f32(a, L, R, O, i, reps, s);
         if (ms) ms[i] = t;
         if (t < best_ms) {
             best_ms = t;
@@ -467,6 +491,13 @@ void cost_model_fit(const char* db_path, const char* model_path, const xc_train_
             M[(size_t)r * nc + c] = ok ? 1.f : 0.f;
         }
     }
+    m.c_max_macs.assign(nc, -INFINITY);
+    for (uint32_t r = 0; r < B; ++r) {
+        float f[kFeatures];
+        workload_features(t.w[rows[r]], f, kFeatures);
+        for (uint32_t c = 0; c < nc; ++c)
+            if (M[(size_t)r * nc + c] > 0.f) m.c_max_macs[c] = std::max(m.c_max_macs[c], (double)f[5]);
+    }
     m.params.resize(mlp_num_params(m.shape));
     mlp_init(m.shape, o.seed, m.params.data());
     std::vector<float> loss(o.epochs);
@@ -474,7 +505,7 @@ void cost_model_fit(const char* db_path, const char* model_path, const xc_train_
 
     std::ofstream out(model_path);
     if (!out) fail(XC_ERR_INVALID_ARGUMENT, std::string("cannot write ") + model_path);
-    out << "xcorr-cost-model 1\n";
+    out << "xcorr-cost-model 2\n";
     out << "candidates " << nc;
     for (const auto& c : m.cands) out << ' ' << c;
     out << '\n';
@@ -487,6 +518,7 @@ void cost_model_fit(const char* db_path, const char* model_path, const xc_train_
     write_list(out, "feat_max", m.f_max);
     write_list(out, "out_mean", m.y_mean);
     write_list(out, "out_std", m.y_std);
+    write_list(out, "cand_max_log2_macs", m.c_max_macs);
     write_list(out, "params", std::vector<double>(m.params.begin(), m.params.end()));
     char buf[160];
     std::snprintf(buf, sizeof(buf), "trained_rows %u holdout_mod %u epochs %u lr %.6g seed %u final_loss %.6g\n", B,

@@ -216,6 +216,10 @@ xc_status xc_workload_features(const xc_workload* w, float* features, uint32_t c
 xc_status xc_autotune_f32(xc_family family, const float* left, const float* right, float* out, uint32_t rows,
                           uint32_t cols, uint32_t n, uint32_t m, uint32_t reps, float* ms, uint32_t* best,
                           xc_stream_t stream);
+/* Time one candidate the way xc_autotune_f32 does (INFINITY if it cannot run the workload). */
+xc_status xc_time_candidate_f32(xc_family family, const float* left, const float* right, float* out, uint32_t rows,
+                                uint32_t cols, uint32_t n, uint32_t m, uint32_t index, uint32_t reps, float* ms,
+                                xc_stream_t stream);
 xc_status xc_autotune_clear(void);
 /* Load a cost-model file (NULL: the default path).  XC_ERR_UNSUPPORTED if it is missing
  * or was trained for another candidate list. */
@@ -226,7 +230,9 @@ xc_status xc_cost_model_predict(const xc_workload* w, float* ms);
 xc_status xc_select(const xc_workload* w, xc_options* chosen, int* source);
 /* Fit the cost model to a timing table written by the tuning collector (CSV: family, rows,
  * cols, n, m, then one column of ms per candidate) on the GPU and write the model file.
- * Rows i with holdout_mod > 0 and i % holdout_mod == 0 are left out of training. */
+ * Rows i with holdout_mod > 0 and i % holdout_mod == 0 are left out of training.  Empty
+ * or "nan" cells are unmeasured candidates; AUTO considers a candidate only up to twice the
+ * largest work (multiply-adds) it has a measured time for. */
 typedef struct xc_train_options {
     uint32_t epochs;
     float lr;
