@@ -185,13 +185,17 @@ def main():
 
         res = {}
         plan = x.BatchPlan(pairs, precision="bf16x3")
+        fplan = x.BatchPlan(pairs, precision="fft64", tile=1)
         for nm, fn in [("batch_bf16x3_graph", lambda: plan.run(L, R, O, use_graph=True)),
-                       ("batch_bf16x3_launch", lambda: plan.run(L, R, O, use_graph=False))]:
+                       ("batch_bf16x3_launch", lambda: plan.run(L, R, O, use_graph=False)),
+                       ("batch_fft64k_graph", lambda: fplan.run(L, R, O, use_graph=True)),
+                       ("batch_fft64k_stream", lambda: fplan.run(L, R, O, use_graph=False))]:
             O.zero_()
             ms = timed(fn, args.steps, args.warmup)
             res[nm] = {"ms": ms, "pairs_per_s": 512 / (ms * 1e-3), "max_norm_err": err_of(), "launches": 1}
             log(f"c4 {nm}: {ms:.4f} ms err {res[nm]['max_norm_err']:.2e}")
         plan.close()
+        fplan.close()
         for m in ["fp32", "bf16x3", "fft64k", "auto"]:
             prec, tile = METH[m]
 
@@ -213,7 +217,9 @@ def main():
                 "methods": res, "auto_picks_per_group": picks,
                 "auto_vs_fp32_direct": fp32 / res["per_class_auto"]["ms"],
                 "batch_graph_vs_per_class_bf16x3": res["per_class_bf16x3"]["ms"] / res["batch_bf16x3_graph"]["ms"],
-                "batch_graph_vs_fp32_direct": fp32 / res["batch_bf16x3_graph"]["ms"]}
+                "batch_graph_vs_fp32_direct": fp32 / res["batch_bf16x3_graph"]["ms"],
+                "fft_batch_graph_vs_fp32_direct": fp32 / res["batch_fft64k_graph"]["ms"],
+                "fft_batch_graph_vs_per_class_fft64k": res["per_class_fft64k"]["ms"] / res["batch_fft64k_graph"]["ms"]}
         print(json.dumps(line), flush=True)
 
     if "c5" in cfgs:

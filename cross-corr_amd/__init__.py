@@ -235,11 +235,13 @@ def batch(pairs, left_flat, right_flat, out_flat, precision="auto", stream=None)
 
 
 class BatchPlan:
-    """Reusable uneven-batch plan (persistent kernel, replayed as a HIP graph)."""
+    """Reusable uneven-batch plan, replayed as a HIP graph."""
 
-    def __init__(self, pairs, precision="bf16x3", border=0):
+    def __init__(self, pairs, precision="bf16x3", border=0, tile=0):
+        """precision bf16x3 / bf16 / fp16: one persistent MFMA kernel; fft64 with tile=1: the
+        own FFT kernels per size class, on parallel graph branches."""
         arr = (Pair * len(pairs))(*[Pair(*p) for p in pairs])
-        opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, 0, 0, border)
+        opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, tile, 0, border)
         self._h = ctypes.c_void_p()
         check(lib().xc_batch_plan_create_f32(arr, len(pairs), ctypes.byref(opt), ctypes.byref(self._h)))
 

@@ -1,5 +1,6 @@
-// Uneven batch: pairs of different sizes in one persistent launch, captured once into a HIP
-// graph and replayed.  No counterpart in the reference, whose n_to_mn wrapper launches one
+// Uneven batch: pairs of different sizes in one launch of a captured HIP graph, replayed.  The
+// graph holds either one persistent MFMA kernel over all pairs (bf16x3 / bf16 / fp16) or, for
+// FFT64 tile 1, the own-FFT kernels of every size class on parallel branches.  No counterpart in the reference, whose n_to_mn wrapper launches one
 // kernel per left matrix on round-robin streams (src/n_to_mn.hpp:553-571).
 #include <hip/hip_runtime.h>
 
@@ -33,14 +34,29 @@ struct xc_batch_plan {
     float* O = nullptr;
     std::vector<xc_pair> pairs;
 
+    // FFT plans: one own-FFT job per size class, classes dealt to parallel graph branches
+    // (each with its own stream and workspace).
+    bool fft = false;
+    std::vector<xc::fftk_job> jobs;
+    std::vector<uint32_t> job_branch;
+    std::vector<hipStream_t> branch_streams;
+    std::vector<xc::fftk_ws> branch_ws;
+    std::vector<void*> allocations;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;
+
     ~xc_batch_plan() {
         // work launched from this plan may still be queued: let the device drain first
         (void)hipDeviceSynchronize();
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (capture_stream) (void)hipStreamDestroy(capture_stream);
+        for (hipStream_t b : branch_streams) (void)hipStreamDestroy(b);
+        if (fork) (void)hipEventDestroy(fork);
+        for (hipEvent_t e : joins) (void)hipEventDestroy(e);
         for (void* p : {(void*)d_items, (void*)d_pairs, (void*)d_tables, (void*)d_counters})
             if (p) (void)hipFree(p);
+        for (void* p : allocations) (void)hipFree(p);
     }
 };
 
@@ -69,11 +85,162 @@ double tile_cost(uint32_t H, uint32_t W, uint32_t tx, uint32_t ty) {
 
 }  // namespace
 
+namespace {
+
+void* plan_alloc(xc_batch_plan* plan, size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(XC_ERR_NO_MEMORY, "batch plan allocation failed");
+    }
+    plan->allocations.push_back(p);
+    return p;
+}
+
+// Own-FFT plan: pairs grouped by size class; per class the distinct left and right matrices
+// (by offset) are transformed once and every pair of the class is one result of the job.
+xc_batch_plan* fft_plan_create(const xc_pair* pairs, uint32_t num_pairs) {
+    auto plan = std::make_unique<xc_batch_plan>();
+    plan->fft = true;
+    plan->kind = XC_PREC_FFT64;
+    plan->n_items = num_pairs;
+    plan->pairs.assign(pairs, pairs + num_pairs);
+    XC_HIP(hipGetDevice(&plan->device));
+    fftk_prepare();
+    struct cls {
+        std::vector<uint64_t> loffs, roffs, ooffs;
+        std::vector<uint32_t> ptab;
+        std::map<uint64_t, uint32_t> lidx, ridx;
+    };
+    std::map<std::pair<uint32_t, uint32_t>, cls> classes;
+    for (uint32_t i = 0; i < num_pairs; ++i) {
+        const xc_pair& p = pairs[i];
+        if (2 * p.rows - 1 > 4096 || 2 * p.cols - 1 > 4096)
+            fail(XC_ERR_UNSUPPORTED, "pair " + std::to_string(i) + ": the FFT batch plan takes rows, cols <= 2048");
+        cls& c = classes[{p.rows, p.cols}];
+        auto li = c.lidx.emplace(p.left_off, (uint32_t)c.loffs.size());
+        if (li.second) c.loffs.push_back(p.left_off);
+        auto ri = c.ridx.emplace(p.right_off, (uint32_t)c.roffs.size());
+        if (ri.second) c.roffs.push_back(p.right_off);
+        c.ptab.push_back(li.first->second);
+        c.ptab.push_back(ri.first->second);
+        c.ooffs.push_back(p.out_off);
+    }
+    // One device buffer for every class's tables.
+    size_t words64 = 0, words32 = 0;
+    for (auto& kv : classes) {
+        words64 += kv.second.loffs.size() + kv.second.roffs.size() + kv.second.ooffs.size();
+        words32 += kv.second.ptab.size();
+    }
+    std::vector<uint64_t> t64;
+    std::vector<uint32_t> t32;
+    t64.reserve(words64);
+    t32.reserve(words32);
+    struct span {
+        size_t moff, ooff, ptab;
+    };
+    std::vector<span> spans;
+    std::vector<double> cost;
+    for (auto& kv : classes) {
+        cls& c = kv.second;
+        span sp{t64.size(), 0, t32.size()};
+        t64.insert(t64.end(), c.loffs.begin(), c.loffs.end());
+        t64.insert(t64.end(), c.roffs.begin(), c.roffs.end());
+        sp.ooff = t64.size();
+        t64.insert(t64.end(), c.ooffs.begin(), c.ooffs.end());
+        t32.insert(t32.end(), c.ptab.begin(), c.ptab.end());
+        spans.push_back(sp);
+        fftk_job j{};
+        j.H = kv.first.first;
+        j.W = kv.first.second;
+        j.nl = (uint32_t)c.loffs.size();
+        j.nr = (uint32_t)c.roffs.size();
+        j.pairs = (uint32_t)c.ooffs.size();
+        plan->jobs.push_back(j);
+        // work estimate: FFT points transformed
+        cost.push_back((double)(j.nl + j.nr + 2.0 * j.pairs) * 4.0 * j.H * j.W);
+    }
+    uint64_t* d64 = static_cast<uint64_t*>(plan_alloc(plan.get(), t64.size() * sizeof(uint64_t)));
+    uint32_t* d32 = static_cast<uint32_t*>(plan_alloc(plan.get(), t32.size() * sizeof(uint32_t)));
+    XC_HIP(hipMemcpy(d64, t64.data(), t64.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    XC_HIP(hipMemcpy(d32, t32.data(), t32.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    for (size_t c = 0; c < plan->jobs.size(); ++c) {
+        plan->jobs[c].moff = d64 + spans[c].moff;
+        plan->jobs[c].ooff = d64 + spans[c].ooff;
+        plan->jobs[c].ptab = d32 + spans[c].ptab;
+    }
+    // Classes, largest first, to the least loaded of up to 8 branches.
+    const uint32_t nb = (uint32_t)std::min<size_t>(8, plan->jobs.size());
+    std::vector<size_t> order(plan->jobs.size());
+    for (size_t c = 0; c < order.size(); ++c) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<double> load(nb, 0.0);
+    plan->job_branch.assign(plan->jobs.size(), 0);
+    std::vector<fftk_ws_sizes> need(nb, fftk_ws_sizes{16, 16, 16, 16, 1});
+    for (size_t c : order) {
+        const uint32_t b = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        plan->job_branch[c] = b;
+        load[b] += cost[c];
+        const fftk_ws_sizes z = fftk_sizes(plan->jobs[c]);
+        need[b].sr = std::max(need[b].sr, z.sr);
+        need[b].sl = std::max(need[b].sl, z.sl);
+        need[b].lh = std::max(need[b].lh, z.lh);
+        need[b].z = std::max(need[b].z, z.z);
+    }
+    for (uint32_t b = 0; b < nb; ++b) {
+        fftk_ws w{};
+        w.sr = plan_alloc(plan.get(), need[b].sr);
+        w.sl = plan_alloc(plan.get(), need[b].sl);
+        w.lh = plan_alloc(plan.get(), need[b].lh);
+        w.z = plan_alloc(plan.get(), need[b].z);
+        w.zchunk = 0;  // per job below
+        plan->branch_ws.push_back(w);
+        hipStream_t st = nullptr;
+        XC_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        plan->branch_streams.push_back(st);
+        hipEvent_t e = nullptr;
+        XC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        plan->joins.push_back(e);
+    }
+    XC_HIP(hipEventCreateWithFlags(&plan->fork, hipEventDisableTiming));
+    XC_HIP(hipStreamCreateWithFlags(&plan->capture_stream, hipStreamNonBlocking));
+    return plan.release();
+}
+
+// Enqueues every class of an FFT plan: on `s` alone, or forked over the branch streams.
+void fft_plan_enqueue(xc_batch_plan* plan, const float* L, const float* R, float* O, hipStream_t s, bool branches) {
+    if (branches) XC_HIP(hipEventRecord(plan->fork, s));
+    std::vector<bool> used(plan->branch_streams.size(), false);
+    for (size_t c = 0; c < plan->jobs.size(); ++c) {
+        const uint32_t b = plan->job_branch[c];
+        hipStream_t st = branches ? plan->branch_streams[b] : s;
+        if (branches && !used[b]) {
+            XC_HIP(hipStreamWaitEvent(st, plan->fork, 0));
+            used[b] = true;
+        }
+        fftk_ws w = plan->branch_ws[b];
+        w.zchunk = fftk_sizes(plan->jobs[c]).chunk;
+        fftk_launch_f32(plan->jobs[c], L, R, O, w, st);
+    }
+    if (branches)
+        for (size_t b = 0; b < used.size(); ++b)
+            if (used[b]) {
+                XC_HIP(hipEventRecord(plan->joins[b], plan->branch_streams[b]));
+                XC_HIP(hipStreamWaitEvent(s, plan->joins[b], 0));
+            }
+}
+
+}  // namespace
+
 xc_batch_plan* batch_plan_create(const xc_pair* pairs, uint32_t num_pairs, const xc_options& opt) {
     check_pairs(pairs, num_pairs);
+    if (opt.precision == XC_PREC_FFT64) {
+        if (opt.tile != 1) fail(XC_ERR_INVALID_ARGUMENT, "FFT batch plans run the own FFT kernels (tile 1)");
+        return fft_plan_create(pairs, num_pairs);
+    }
     const int kind = opt.precision == XC_PREC_AUTO ? XC_PREC_BF16X3 : (int)opt.precision;
     if (kind != XC_PREC_BF16X3 && kind != XC_PREC_BF16 && kind != XC_PREC_FP16)
-        fail(XC_ERR_INVALID_ARGUMENT, "batch plans run the MFMA methods (bf16x3, bf16, fp16)");
+        fail(XC_ERR_INVALID_ARGUMENT, "batch plans run the MFMA methods (bf16x3, bf16, fp16) or FFT64 tile 1");
     if (opt.border > 512) fail(XC_ERR_INVALID_ARGUMENT, "border threshold above 512");
     const uint32_t thresh = kind == XC_PREC_BF16X3 ? (opt.border ? opt.border : batch_default_border()) : 0;
 
@@ -157,6 +324,42 @@ void batch_plan_exec(xc_batch_plan* plan, const float* L, const float* R, float*
     int dev = 0;
     XC_HIP(hipGetDevice(&dev));
     if (dev != plan->device) fail(XC_ERR_INVALID_ARGUMENT, "batch plan used on another device than it was made on");
+    if (plan->fft) {
+        if (!use_graph) {
+            fft_plan_enqueue(plan, L, R, O, s, false);
+            return;
+        }
+        if (plan->exec && (plan->L != L || plan->R != R || plan->O != O)) {
+            // the graph bakes the matrix pointers into every node: capture it again
+            XC_HIP(hipGraphExecDestroy(plan->exec));
+            XC_HIP(hipGraphDestroy(plan->graph));
+            plan->exec = nullptr;
+            plan->graph = nullptr;
+        }
+        if (!plan->exec) {
+            XC_HIP(hipStreamBeginCapture(plan->capture_stream, hipStreamCaptureModeThreadLocal));
+            try {
+                fft_plan_enqueue(plan, L, R, O, plan->capture_stream, true);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(plan->capture_stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hipGraph_t g = nullptr;
+            XC_HIP(hipStreamEndCapture(plan->capture_stream, &g));
+            plan->graph = g;
+            XC_HIP(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+            plan->L = L;
+            plan->R = R;
+            plan->O = O;
+        }
+        record_launch("fft batch graph", dim3((uint32_t)plan->jobs.size()), dim3(256), 0, 0, 0,
+                      (uint32_t)plan->pairs.size(), (uint32_t)plan->branch_streams.size());
+        timed tg("batch_graph", s);
+        XC_HIP(hipGraphLaunch(plan->exec, s));
+        return;
+    }
     uint32_t n_items = plan->n_items, thresh = plan->thresh;
     void* args[] = {&plan->d_items, &n_items, &plan->d_pairs, &plan->d_tables, &L, &R, &O, &plan->d_counters, &thresh};
     record_launch("batch_kernel (persistent)", dim3(plan->grid), dim3(256), plan->lds, 0, 0,
@@ -208,9 +411,18 @@ void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, con
         }
         return;
     }
+    xc_options o = opt;
+    if (o.precision == XC_PREC_AUTO) {
+        // The own FFT method is the faster one at every size class it takes (rows, cols <= 2048).
+        bool fits = true;
+        for (uint32_t i = 0; i < num_pairs && fits; ++i) fits = pairs[i].rows <= 2048 && pairs[i].cols <= 2048;
+        o = xc_options{};
+        o.precision = fits ? XC_PREC_FFT64 : XC_PREC_BF16X3;
+        o.tile = fits ? 1 : 0;
+    }
     static std::mutex mu;
     static std::list<std::pair<std::string, std::unique_ptr<xc_batch_plan>>> cache;
-    std::string key(reinterpret_cast<const char*>(&opt), sizeof(opt));
+    std::string key(reinterpret_cast<const char*>(&o), sizeof(o));
     key.append(reinterpret_cast<const char*>(pairs), sizeof(xc_pair) * num_pairs);
     int dev = 0;
     XC_HIP(hipGetDevice(&dev));
@@ -224,7 +436,7 @@ void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, con
             break;
         }
     if (!plan) {
-        cache.emplace_front(key, std::unique_ptr<xc_batch_plan>(batch_plan_create(pairs, num_pairs, opt)));
+        cache.emplace_front(key, std::unique_ptr<xc_batch_plan>(batch_plan_create(pairs, num_pairs, o)));
         plan = cache.front().second.get();
         if (cache.size() > 8) cache.pop_back();
     }

@@ -194,12 +194,13 @@ __device__ __forceinline__ void fft_slab(double2* buf, int T, const double2* __r
 }
 
 // rows_fwd: block (matrix, group of 2T rows); matrices 0..nl-1 are the left ones (into SL),
-// the rest the right ones (into SR).
+// the rest the right ones (into SR).  Matrix i starts at element moff[i] of its input when
+// the offset table is given (uneven batches), else at i * H * W.
 template <typename Tin, int LOGPC>
 __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ XL, const Tin* __restrict__ XR,
-                                                       uint32_t nl, uint32_t H, uint32_t W, int logT, uint32_t groups,
-                                                       double2* __restrict__ SL, double2* __restrict__ SR,
-                                                       const double2* __restrict__ tw) {
+                                                       const uint64_t* __restrict__ moff, uint32_t nl, uint32_t H,
+                                                       uint32_t W, int logT, uint32_t groups, double2* __restrict__ SL,
+                                                       double2* __restrict__ SR, const double2* __restrict__ tw) {
     constexpr int logPc = LOGPC;
     extern __shared__ double2 buf[];
     const uint32_t matg = blockIdx.x / groups, g = blockIdx.x - matg * groups;
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
     const uint32_t mat = left ? matg : matg - nl;
     const int Pc = 1 << logPc, T = 1 << logT;
     const uint32_t r0 = g << (logT + 1);
-    const Tin* Xm = (left ? XL : XR) + (size_t)mat * H * W;
+    const Tin* Xm = (left ? XL : XR) + (moff ? moff[matg] : (size_t)mat * H * W);
     double2* S = left ? SL : SR;
     for (int idx = threadIdx.x; idx < (T << logPc); idx += kFT) {
         const int t = idx >> logPc, c = idx & (Pc - 1);
@@ -241,15 +242,16 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
 
 // cols: block (item, group of C columns k).  mode 0: item = left matrix l, writes its full
 // column spectra Lhat[l][k][rho] (the FFT along rho of SL[l][k][0..H), zero padded).
-// mode 1: item = pair q0 + item (left l, right r): X = the same FFT of SR[r][k], then a
+// mode 1: item = pair q0 + item (left l, right r from the family's order, or from the pair
+// table of an uneven batch): X = the same FFT of SR[r][k], then a
 // second FFT of conj(Lhat) * X, conjugated: Z[item][k][rho] = Pr * IFFT_rho(Lhat * conj(X)).
 // Blocks are ordered column-group major through the XCD remap, so the pairs that read the
 // same Lhat columns run on one XCD and find them in its L2.
 template <int LOGPR>
 __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S, double2* __restrict__ Lhat,
-                                                   double2* __restrict__ Z, int mode, int family, uint32_t n,
-                                                   uint32_t m, uint32_t q0, uint32_t items, uint32_t H, uint32_t Kc,
-                                                   int logC, const double2* __restrict__ tw) {
+                                                   double2* __restrict__ Z, const uint2* __restrict__ ptab, int mode,
+                                                   int family, uint32_t n, uint32_t m, uint32_t q0, uint32_t items,
+                                                   uint32_t H, uint32_t Kc, int logC, const double2* __restrict__ tw) {
     constexpr int logPr = LOGPR;
     extern __shared__ double2 buf[];
     const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -257,7 +259,14 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
     const int Pr = 1 << logPr, C = 1 << logC;
     const uint32_t k0 = cg << logC;
     uint32_t l = item, r = item;
-    if (mode == 1) pair_of(family, n, m, q0 + item, l, r);
+    if (mode == 1) {
+        if (ptab) {
+            l = ptab[q0 + item].x;
+            r = ptab[q0 + item].y;
+        } else {
+            pair_of(family, n, m, q0 + item, l, r);
+        }
+    }
     const double2* Sm = S + (size_t)r * Kc * H;
     for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
         const uint32_t k = k0 + (idx >> logPr), rho = idx & (Pr - 1);
@@ -300,11 +309,13 @@ __device__ __forceinline__ int out_row(uint32_t rho, uint32_t H, uint32_t Pr) {
 // rows_inv: block (pair item, group of 2T consecutive circular rows rho0 = 2T*g ...), so that
 // each thread's two rows are adjacent in Z and a group's rows are one aligned run.  Row rho
 // goes to output row out_row(rho); output column ox = dx + W - 1 takes circular column
-// dx mod Pc.
+// dx mod Pc.  Pair q's result starts at ooff[q] when the table is given, else at q times the
+// result size.
 template <typename Tout, int LOGPC>
 __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict__ Z, Tout* __restrict__ O,
-                                                       uint32_t q0, uint32_t H, uint32_t W, int logPr, int logT,
-                                                       uint32_t groups, double scale, const double2* __restrict__ tw) {
+                                                       const uint64_t* __restrict__ ooff, uint32_t q0, uint32_t H,
+                                                       uint32_t W, int logPr, int logT, uint32_t groups, double scale,
+                                                       const double2* __restrict__ tw) {
     constexpr int logPc = LOGPC;
     extern __shared__ double2 buf[];
     const uint32_t item = blockIdx.x / groups, g = blockIdx.x - item * groups;
@@ -331,7 +342,7 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
     __syncthreads();
     fft_slab<LOGPC>(buf, T, tw);
     // IFFT(Y) = conj(FFT(conj(Y))) / Pc: row A is the real part, row B minus the imaginary.
-    Tout* Oq = O + (size_t)(q0 + item) * (2 * H - 1) * SX;
+    Tout* Oq = O + (ooff ? ooff[q0 + item] : (size_t)(q0 + item) * (2 * H - 1) * SX);
     const int lane = threadIdx.x & 63;
     for (int rr = threadIdx.x >> 6; rr < rows2; rr += kFT / 64) {
         const int oy = out_row(rho0 + rr, H, Pr);
@@ -437,77 +448,129 @@ void allow_lds(K kernel, uint32_t bytes) {
                                    bytes));
 }
 
-template <typename Tio>
-void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, hipStream_t s) {
-    const uint32_t H = a.rows, W = a.cols;
-    if (2 * H - 1 > (uint32_t)kSlab || 2 * W - 1 > (uint32_t)kSlab)
-        fail(XC_ERR_UNSUPPORTED, "own-FFT method: rows and cols must be at most 2048");
-    const int logPr = std::max(4, ilog2_ceil(2 * H - 1)), logPc = std::max(4, ilog2_ceil(2 * W - 1));
-    const uint32_t Pr = 1u << logPr, Pc = 1u << logPc, Kc = Pc / 2 + 1;
-    const uint32_t nl = a.family == XC_ONE_TO_ONE || a.family == XC_ONE_TO_MANY ? 1 : a.n;
-    const uint32_t nr = a.family == XC_ONE_TO_ONE ? 1 : a.m;
-    const uint32_t pairs = a.pairs;
-    const double2* tw = twiddles();
+struct fftk_geom {
+    int logPr, logPc;
+    uint32_t Pr, Pc, Kc;
+    slab_shape v1, vc, v3;
+    uint32_t g1, cg, g3;
+    size_t s_bytes, col_bytes;
+};
 
+fftk_geom geom_of(uint32_t H, uint32_t W) {
+    if (H == 0 || W == 0 || 2 * H - 1 > (uint32_t)kSlab || 2 * W - 1 > (uint32_t)kSlab)
+        fail(XC_ERR_UNSUPPORTED, "own-FFT method: rows and cols must be 1..2048");
+    fftk_geom g{};
+    g.logPr = std::max(4, ilog2_ceil(2 * H - 1));
+    g.logPc = std::max(4, ilog2_ceil(2 * W - 1));
+    g.Pr = 1u << g.logPr;
+    g.Pc = 1u << g.logPc;
+    g.Kc = g.Pc / 2 + 1;
     // Slab shape per kernel (see shape_for): rows_fwd 2T input rows per block, cols C columns,
     // rows_inv 2T output rows.
-    const slab_shape v1 = shape_for(logPc, (H + 1) / 2);
-    const slab_shape vc = shape_for(logPr, Kc);
-    const slab_shape v3 = shape_for(logPc, Pr / 2);  // 2T <= Pr
-    const int logT1 = v1.logT, logC = vc.logT, logT3 = v3.logT;
-    const uint32_t g1 = (H + (2u << logT1) - 1) >> (logT1 + 1);
-    const uint32_t cg = (Kc + (1u << logC) - 1) >> logC;
-    const uint32_t g3 = Pr >> (logT3 + 1);
+    g.v1 = shape_for(g.logPc, (H + 1) / 2);
+    g.vc = shape_for(g.logPr, g.Kc);
+    g.v3 = shape_for(g.logPc, g.Pr / 2);  // 2T <= Pr
+    g.g1 = (H + (2u << g.v1.logT) - 1) >> (g.v1.logT + 1);
+    g.cg = (g.Kc + (1u << g.vc.logT) - 1) >> g.vc.logT;
+    g.g3 = g.Pr >> (g.v3.logT + 1);
+    g.s_bytes = sizeof(double2) * (size_t)g.Kc * H;
+    g.col_bytes = sizeof(double2) * (size_t)g.Kc * g.Pr;
+    return g;
+}
 
-    const size_t s_bytes = sizeof(double2) * (size_t)Kc * H;
-    const size_t col_bytes = sizeof(double2) * (size_t)Kc * Pr;
-    double2* SR = static_cast<double2*>(workspace(WS_FK_SR, s_bytes * nr, s));
-    double2* SL = static_cast<double2*>(workspace(WS_FK_SL, s_bytes * nl, s));
-    double2* LH = static_cast<double2*>(workspace(WS_FK_LHAT, col_bytes * nl, s));
-    // Z is processed in chunks of pairs of at most ~2 GiB.
-    const uint32_t chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(pairs, (size_t(2) << 30) / col_bytes));
-    double2* Zw = static_cast<double2*>(workspace(WS_FK_Z, col_bytes * chunk, s));
-
+template <typename Tio>
+void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fftk_ws& ws, hipStream_t s,
+                bool timing) {
+    const uint32_t H = j.H, W = j.W;
+    const fftk_geom g = geom_of(H, W);
+    const double2* tw = twiddles();
+    double2* SR = static_cast<double2*>(ws.sr);
+    double2* SL = static_cast<double2*>(ws.sl);
+    double2* LH = static_cast<double2*>(ws.lh);
+    double2* Zw = static_cast<double2*>(ws.z);
+    const uint32_t chunk = std::max<uint32_t>(1, std::min<uint32_t>(j.pairs, ws.zchunk));
+    const uint2* ptab = reinterpret_cast<const uint2*>(j.ptab);
+    auto scope = [&](const char* name) { return timed(name, timing ? s : nullptr, timing); };
     {
-        timed t0("fftk_rows_fwd", s);
-        with_logp(logPc, [&](auto LP) {
+        auto t0 = scope("fftk_rows_fwd");
+        with_logp(g.logPc, [&](auto LP) {
             auto k = rows_fwd_kernel<Tio, decltype(LP)::value>;
-            allow_lds(k, v1.lds);
-            hipLaunchKernelGGL(k, dim3((nl + nr) * g1), dim3(kFT), v1.lds, s, L, R, nl, H, W, logT1, g1, SL, SR, tw);
+            allow_lds(k, g.v1.lds);
+            hipLaunchKernelGGL(k, dim3((j.nl + j.nr) * g.g1), dim3(kFT), g.v1.lds, s, L, R, j.moff, j.nl, H, W,
+                               g.v1.logT, g.g1, SL, SR, tw);
         });
     }
     auto cols = [&](const double2* S, double2* Zout, int mode, uint32_t items, uint32_t q0) {
-        with_logp(logPr, [&](auto LP) {
+        with_logp(g.logPr, [&](auto LP) {
             auto k = cols_kernel<decltype(LP)::value>;
-            allow_lds(k, vc.lds);
-            hipLaunchKernelGGL(k, dim3(items * cg), dim3(kFT), vc.lds, s, S, LH, Zout, mode, a.family, a.n, a.m, q0,
-                               items, H, Kc, logC, tw);
+            allow_lds(k, g.vc.lds);
+            hipLaunchKernelGGL(k, dim3(items * g.cg), dim3(kFT), g.vc.lds, s, S, LH, Zout, ptab, mode, j.family, j.n,
+                               j.m, q0, items, H, g.Kc, g.vc.logT, tw);
         });
     };
     {
-        timed t1("fftk_cols_left", s);
-        cols(SL, nullptr, 0, nl, 0);
+        auto t1 = scope("fftk_cols_left");
+        cols(SL, nullptr, 0, j.nl, 0);
     }
-    const double scale = 1.0 / ((double)Pr * (double)Pc);
-    for (uint32_t q0 = 0; q0 < pairs; q0 += chunk) {
-        const uint32_t nq = std::min(chunk, pairs - q0);
+    const double scale = 1.0 / ((double)g.Pr * (double)g.Pc);
+    for (uint32_t q0 = 0; q0 < j.pairs; q0 += chunk) {
+        const uint32_t nq = std::min(chunk, j.pairs - q0);
         {
-            timed t2("fftk_cols", s);
+            auto t2 = scope("fftk_cols");
             cols(SR, Zw, 1, nq, q0);
         }
-        timed t3("fftk_rows_inv", s);
-        with_logp(logPc, [&](auto LP) {
+        auto t3 = scope("fftk_rows_inv");
+        with_logp(g.logPc, [&](auto LP) {
             auto k = rows_inv_kernel<Tio, decltype(LP)::value>;
-            allow_lds(k, v3.lds);
-            hipLaunchKernelGGL(k, dim3(nq * g3), dim3(kFT), v3.lds, s, (const double2*)Zw, O, q0, H, W, logPr, logT3,
-                               g3, scale, tw);
+            allow_lds(k, g.v3.lds);
+            hipLaunchKernelGGL(k, dim3(nq * g.g3), dim3(kFT), g.v3.lds, s, (const double2*)Zw, O, j.ooff, q0, H, W,
+                               g.logPr, g.v3.logT, g.g3, scale, tw);
         });
     }
     XC_HIP(hipGetLastError());
-    record_launch("fftk_rows_inv", dim3(g3), dim3(kFT), v3.lds, H, W, pairs, 1);
+    record_launch("fftk_rows_inv", dim3(g.g3), dim3(kFT), g.v3.lds, H, W, j.pairs, 1);
+}
+
+template <typename Tio>
+void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, hipStream_t s) {
+    fftk_job j{};
+    j.H = a.rows;
+    j.W = a.cols;
+    j.nl = a.family == XC_ONE_TO_ONE || a.family == XC_ONE_TO_MANY ? 1 : a.n;
+    j.nr = a.family == XC_ONE_TO_ONE ? 1 : a.m;
+    j.pairs = a.pairs;
+    j.family = a.family;
+    j.n = a.n;
+    j.m = a.m;
+    const fftk_ws_sizes z = fftk_sizes(j);
+    fftk_ws ws{};
+    ws.sr = workspace(WS_FK_SR, z.sr, s);
+    ws.sl = workspace(WS_FK_SL, z.sl, s);
+    ws.lh = workspace(WS_FK_LHAT, z.lh, s);
+    ws.z = workspace(WS_FK_Z, z.z, s);
+    ws.zchunk = z.chunk;
+    launch_job<Tio>(j, L, R, O, ws, s, true);
 }
 
 }  // namespace
+
+fftk_ws_sizes fftk_sizes(const fftk_job& j) {
+    const fftk_geom g = geom_of(j.H, j.W);
+    fftk_ws_sizes z{};
+    // Z is processed in chunks of pairs of at most ~2 GiB.
+    z.chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(j.pairs, (size_t(2) << 30) / g.col_bytes));
+    z.sr = std::max<size_t>(g.s_bytes * j.nr, 16);
+    z.sl = std::max<size_t>(g.s_bytes * j.nl, 16);
+    z.lh = std::max<size_t>(g.col_bytes * j.nl, 16);
+    z.z = g.col_bytes * z.chunk;
+    return z;
+}
+
+void fftk_prepare() { (void)twiddles(); }
+
+void fftk_launch_f32(const fftk_job& j, const float* L, const float* R, float* O, const fftk_ws& ws, hipStream_t s) {
+    launch_job<float>(j, L, R, O, ws, s, false);
+}
 
 void run_fftk_f32(const family_args& a, const float* L, const float* R, float* O, hipStream_t s) {
     run_fftk<float>(a, L, R, O, s);

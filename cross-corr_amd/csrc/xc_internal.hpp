@@ -41,12 +41,13 @@ struct timed {
     const char* name;
     hipStream_t s;
     bool on;
-    timed(const char* n, hipStream_t st) : name(n), s(st), on(timing_enabled()) {
+    timed(const char* n, hipStream_t st, bool allow = true) : name(n), s(st), on(allow && timing_enabled()) {
         if (on) timing_mark(name, s, 0);
     }
     ~timed() {
         if (on) timing_mark(name, s, 1);
     }
+    timed(timed&& o) noexcept : name(o.name), s(o.s), on(o.on) { o.on = false; }
     timed(const timed&) = delete;
     timed& operator=(const timed&) = delete;
 };

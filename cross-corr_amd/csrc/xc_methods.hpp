@@ -55,6 +55,30 @@ void run_fft_f32(const family_args& a, const float* L, const float* R, float* O,
                  bool fp64_transform, hipStream_t s);
 void run_fft_f64(const family_args& a, const double* L, const double* R, double* O, bool normalize,
                  hipStream_t s);
+// One launch sequence of the own-FFT method: nl left and nr right matrices of H x W and
+// `pairs` results.  The device tables are null for a regular family call; an uneven batch
+// passes, per size class, the element offsets of its matrices (nl left ones, then nr right
+// ones), a (left index, right index) word pair per result and each result's offset.
+struct fftk_job {
+    uint32_t H = 0, W = 0, nl = 0, nr = 0, pairs = 0;
+    int family = 0;
+    uint32_t n = 0, m = 0;
+    const uint64_t* moff = nullptr;
+    const uint32_t* ptab = nullptr;
+    const uint64_t* ooff = nullptr;
+};
+struct fftk_ws_sizes {
+    size_t sr, sl, lh, z;
+    uint32_t chunk;  // pairs per launch of the cols / rows_inv kernels
+};
+struct fftk_ws {
+    void *sr, *sl, *lh, *z;
+    uint32_t zchunk;
+};
+fftk_ws_sizes fftk_sizes(const fftk_job& j);
+void fftk_prepare();  // device tables the kernels read (call before stream capture)
+void fftk_launch_f32(const fftk_job& j, const float* L, const float* R, float* O, const fftk_ws& ws, hipStream_t s);
+
 // FFT method on the library's own FP64 FFT kernels (xc_fftk.hip): rows, cols <= 2048.
 void run_fftk_f32(const family_args& a, const float* L, const float* R, float* O, hipStream_t s);
 void run_fftk_f64(const family_args& a, const double* L, const double* R, double* O, hipStream_t s);

@@ -1,6 +1,7 @@
 """Uneven batch (n_to_mn-style groups of different sizes) through xc_batch_f32 and reusable
-plans: one persistent launch, captured into a HIP graph and replayed.  Every pair is checked
-against the FP64 oracle (bf16x3: <= 2e-6 normalised; fp32 per-pair direct: <= 1e-6)."""
+plans: one persistent MFMA launch, or the own FFT kernels per size class on parallel graph
+branches, captured into a HIP graph and replayed.  Every pair is checked against the FP64
+oracle (bf16x3: <= 2e-6 normalised; fp32 per-pair direct and FFT64: <= 1e-6)."""
 import numpy as np
 import pytest
 
@@ -88,3 +89,94 @@ def test_batch_invalid_inputs_rejected():
         x.BatchPlan([(0, 0, 0, 0, 4)], "bf16x3")
     with pytest.raises(x.XcError):
         x.BatchPlan([(0, 0, 0, 4, 4)], "fft64")
+
+
+def _mixed_pairs(seed):
+    """Pairs of several shapes, some sharing a left or a right matrix, in one flat buffer."""
+    rng = np.random.default_rng(seed)
+    shapes = [(1, 1), (2, 3), (9, 5), (5, 9), (16, 16), (33, 20), (64, 64), (100, 37)]
+    lefts, rights, pairs = [], [], []
+    loff = roff = ooff = 0
+    for (r, c) in shapes:
+        nl, nr = int(rng.integers(1, 3)), int(rng.integers(1, 4))
+        lo = [loff + i * r * c for i in range(nl)]
+        ro = [roff + i * r * c for i in range(nr)]
+        lefts.append(rng.random(nl * r * c).astype(np.float32))
+        rights.append((rng.random(nr * r * c) * 2 - 1).astype(np.float32))
+        for i in range(nl):
+            for j in range(nr):
+                pairs.append((lo[i], ro[j], ooff, r, c))
+                ooff += (2 * r - 1) * (2 * c - 1)
+        loff += nl * r * c
+        roff += nr * r * c
+    return np.concatenate(lefts), np.concatenate(rights), pairs, ooff
+
+
+def test_batch_fft_plan_mixed_shapes_shared_matrices():
+    x = xcorr()
+    lf, rf, pairs, olen = _mixed_pairs(7)
+    plan = x.BatchPlan(pairs, "fft64", tile=1)
+    out = torch.full((olen,), float("nan"), device="cuda")
+    plan.run(dev(lf), dev(rf), out, use_graph=True)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert not np.isnan(got).any()
+    batch_oracle_check(lf, rf, pairs, got, 1e-6)
+    plan.close()
+
+
+def test_batch_fft_plan_replays_recapture_and_eager():
+    x = xcorr()
+    lf, rf, pairs, olen, _ = uneven_batch(12, 8, lo=16, hi=300)
+    plan = x.BatchPlan(pairs, "fft64", tile=1)
+    L, R = dev(lf), dev(rf)
+    outs = [torch.full((olen,), float("nan"), device="cuda") for _ in range(2)]
+    ref = None
+    for rep in range(4):
+        o = outs[rep % 2]  # alternating outputs: the graph is captured again for new pointers
+        o.fill_(float("nan"))
+        plan.run(L, R, o, use_graph=True)
+        torch.cuda.synchronize()
+        _ = (L * 2.0).sum().item()  # eager work between replays
+        got = o.cpu().numpy()
+        assert not np.isnan(got).any(), rep
+        if ref is None:
+            order = np.argsort([p[3] for p in pairs])
+            batch_oracle_check(lf, rf, pairs, got, 1e-6, check_pairs=list(order[:24]))
+            ref = got
+        else:
+            np.testing.assert_array_equal(got, ref)
+    o = torch.full((olen,), float("nan"), device="cuda")
+    plan.run(L, R, o, use_graph=False)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(o.cpu().numpy(), ref)
+    plan.close()
+
+
+def test_batch_auto_is_the_fft_plan_on_the_c4_shape():
+    """C4 shape through xc_batch_f32 with AUTO: oracle on the smallest groups and two near
+    size 128, the sum rule on every pair."""
+    lf, rf, pairs, olen, sizes = uneven_batch(128, 2026)
+    out = torch.full((olen,), float("nan"), device="cuda")
+    x = xcorr()
+    x.batch(pairs, dev(lf), dev(rf), out, precision="auto")
+    torch.cuda.synchronize()
+    assert x.last_launch()["kernel"] == "fft batch graph"
+    got = out.cpu().numpy()
+    assert not np.isnan(got).any()
+    order = np.argsort(sizes)
+    mid = sorted(range(len(sizes)), key=lambda g: abs(sizes[g] - 128))[:2]
+    groups = list(order[:8]) + mid
+    batch_oracle_check(lf, rf, pairs, got, 1e-6, check_pairs=[4 * g + j for g in groups for j in (0, 3)])
+    for lo, ro, oo, r, c in pairs:
+        s = got[oo:oo + (2 * r - 1) * (2 * c - 1)].astype(np.float64).sum()
+        e = lf[lo:lo + r * c].astype(np.float64).sum() * rf[ro:ro + r * c].astype(np.float64).sum()
+        den = np.abs(lf[lo:lo + r * c]).astype(np.float64).sum() * np.abs(rf[ro:ro + r * c]).astype(np.float64).sum()
+        assert abs(s - e) / den < 1e-6
+
+
+def test_batch_fft_plan_rejects_oversize():
+    x = xcorr()
+    with pytest.raises(x.XcError) as e:
+        x.BatchPlan([(0, 0, 0, 2049, 4)], "fft64", tile=1)
+    assert e.value.status == 4
