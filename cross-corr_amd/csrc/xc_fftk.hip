@@ -175,8 +175,9 @@ __device__ __forceinline__ void radix8_passes(double2* buf, int T, const double2
 }
 
 // Forward FFT of T transforms of length 2^LOGP in buf; natural order.  The caller has
-// synchronised the block before; the block is synchronised on return.
-template <int LOGP>
+// synchronised the block before (or, with WAVE_ONLY, the wave: every transform's data was
+// written by the wave that owns it); the block (WAVE_ONLY: the wave) is synchronised on return.
+template <int LOGP, bool WAVE_ONLY = false>
 __device__ __forceinline__ void fft_slab(double2* buf, int T, const double2* __restrict__ tw) {
     constexpr int LOGS = slab_of<LOGP>::LOGS;
     constexpr bool WAVE = slab_of<LOGP>::WAVE;
@@ -190,7 +191,52 @@ __device__ __forceinline__ void fft_slab(double2* buf, int T, const double2* __r
     } else {
         radix8_passes<LOGP, 0>(buf, T, tw);
     }
-    if constexpr (WAVE) __syncthreads();
+    static_assert(!WAVE_ONLY || WAVE, "wave-only synchronisation needs wave-owned transforms");
+    if constexpr (WAVE && !WAVE_ONLY) __syncthreads();
+}
+
+// 512-point FFT owned by one wave, entered and left in registers: lane i holds x[i + 64 r]
+// (r = 0..7) on entry and X[i + 64 r] on return.  The three radix-8 passes exchange data
+// through the wave's LDS area wb (slots at<9>(0, j)) twice; the first pass reads, and the last
+// pass writes, registers only (its outputs i + 64 r are its inputs' positions).  So a kernel
+// that loads x[i + 64 r] from HBM and stores X[i + 64 r] to HBM needs no other LDS traffic.
+__device__ __forceinline__ void fft512_regs(double2 (&u)[8], double2* wb, const double2* __restrict__ tw) {
+    const int i = threadIdx.x & 63;
+    // pass 1 (p = 1): no twiddles; outputs 8 i + r
+    dft<8>(u);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) wb[at<9>(0, 8 * i + r)] = u[r];
+    slab_sync<true>();
+    // pass 2 (p = 8): k = i mod 8; outputs (i - k) * 8 + k + 8 r
+    {
+        const int k = i & 7;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) u[r] = wb[at<9>(0, i + 64 * r)];
+        const double2 w1 = tw[k << 6];
+        double2 w = w1;
+#pragma unroll
+        for (int r = 1; r < 8; ++r) {
+            u[r] = cmul(u[r], w);
+            if (r + 1 < 8) w = cmul(w, w1);
+        }
+        dft<8>(u);
+        slab_sync<true>();
+#pragma unroll
+        for (int r = 0; r < 8; ++r) wb[at<9>(0, ((i - k) << 3) + k + 8 * r)] = u[r];
+        slab_sync<true>();
+    }
+    // pass 3 (p = 64): k = i; outputs i + 64 r, kept in registers
+#pragma unroll
+    for (int r = 0; r < 8; ++r) u[r] = wb[at<9>(0, i + 64 * r)];
+    const double2 w1 = tw[i << 3];
+    double2 w = w1;
+#pragma unroll
+    for (int r = 1; r < 8; ++r) {
+        u[r] = cmul(u[r], w);
+        if (r + 1 < 8) w = cmul(w, w1);
+    }
+    dft<8>(u);
+    slab_sync<true>();  // the area may be written again right after
 }
 
 // rows_fwd: block (matrix, group of 2T rows); matrices 0..nl-1 are the left ones (into SL),
@@ -210,15 +256,33 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
     const uint32_t r0 = g << (logT + 1);
     const Tin* Xm = (left ? XL : XR) + (moff ? moff[matg] : (size_t)mat * H * W);
     double2* S = left ? SL : SR;
-    for (int idx = threadIdx.x; idx < (T << logPc); idx += kFT) {
-        const int t = idx >> logPc, c = idx & (Pc - 1);
+    if constexpr (LOGPC == 9) {
+        // T = 4: wave w transforms rows r0 + 2w, r0 + 2w + 1 in registers (fft512_regs) and
+        // leaves the spectrum in natural order in its LDS area for the block's split below.
+        const int lane = threadIdx.x & 63, t = threadIdx.x >> 6;
         const uint32_t a = r0 + 2 * t, b = a + 1;
-        const double re = (a < H && (uint32_t)c < W) ? (double)Xm[(size_t)a * W + c] : 0.0;
-        const double im = (b < H && (uint32_t)c < W) ? (double)Xm[(size_t)b * W + c] : 0.0;
-        buf[at<LOGPC>(t, c)] = make_double2(re, im);
+        double2 u[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint32_t c = lane + 64 * r;
+            u[r] = make_double2((a < H && c < W) ? (double)Xm[(size_t)a * W + c] : 0.0,
+                                (b < H && c < W) ? (double)Xm[(size_t)b * W + c] : 0.0);
+        }
+        fft512_regs(u, buf + at<9>(t, 0), tw);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) buf[at<9>(t, lane + 64 * r)] = u[r];
+        __syncthreads();
+    } else {
+        for (int idx = threadIdx.x; idx < (T << logPc); idx += kFT) {
+            const int t = idx >> logPc, c = idx & (Pc - 1);
+            const uint32_t a = r0 + 2 * t, b = a + 1;
+            const double re = (a < H && (uint32_t)c < W) ? (double)Xm[(size_t)a * W + c] : 0.0;
+            const double im = (b < H && (uint32_t)c < W) ? (double)Xm[(size_t)b * W + c] : 0.0;
+            buf[at<LOGPC>(t, c)] = make_double2(re, im);
+        }
+        __syncthreads();
+        fft_slab<LOGPC>(buf, T, tw);
     }
-    __syncthreads();
-    fft_slab<LOGPC>(buf, T, tw);
     const uint32_t Kc = (uint32_t)(Pc / 2 + 1);
     const int rows2 = 2 << logT;
     double2* Sm = S + (size_t)mat * Kc * H;
@@ -268,13 +332,75 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
         }
     }
     const double2* Sm = S + (size_t)r * Kc * H;
+    double2* Ll = Lhat + (size_t)l * Kc * Pr;
+    if constexpr (LOGPR == 9) {
+        // C = 4: wave w owns column k0 + w from HBM to HBM in registers (fft512_regs); the
+        // product with conj(Lhat) happens in registers between the two FFTs.
+        const int lane = threadIdx.x & 63, t = threadIdx.x >> 6;
+        const uint32_t k = k0 + t;
+        if (k >= Kc) return;
+        double2* wb = buf + at<9>(t, 0);
+        double2 u[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint32_t rho = lane + 64 * r;
+            u[r] = rho < H ? Sm[(size_t)k * H + rho] : make_double2(0.0, 0.0);
+        }
+        fft512_regs(u, wb, tw);
+        const double2* lk = Ll + (size_t)k * 512;
+        if (mode == 0) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) Ll[(size_t)k * 512 + lane + 64 * r] = u[r];
+            return;
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r) u[r] = cmul(cconj(lk[lane + 64 * r]), u[r]);
+        fft512_regs(u, wb, tw);
+        double2* zk = Z + (size_t)item * Kc * Pr + (size_t)k * 512;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) zk[lane + 64 * r] = cconj(u[r]);
+        return;
+    }
+    if constexpr (slab_of<LOGPR>::WAVE) {
+        // Wave-local: wave w owns columns [w*C/4, (w+1)*C/4) through every phase, so no block
+        // barrier is needed anywhere.
+        const int lane = threadIdx.x & 63, tpw = C >> 2, t0 = (threadIdx.x >> 6) * tpw;
+        const int n_el = tpw << LOGPR;
+        for (int e = lane; e < n_el; e += 64) {
+            const int t = t0 + (e >> LOGPR), rho = e & (Pr - 1);
+            const uint32_t k = k0 + t;
+            buf[at<LOGPR>(t, rho)] = (k < Kc && (uint32_t)rho < H) ? Sm[(size_t)k * H + rho] : make_double2(0.0, 0.0);
+        }
+        slab_sync<true>();
+        fft_slab<LOGPR, true>(buf, C, tw);
+        if (mode == 0) {
+            for (int e = lane; e < n_el; e += 64) {
+                const int t = t0 + (e >> LOGPR), rho = e & (Pr - 1);
+                if (k0 + t < Kc) Ll[(size_t)(k0 + t) * Pr + rho] = buf[at<LOGPR>(t, rho)];
+            }
+            return;
+        }
+        for (int e = lane; e < n_el; e += 64) {
+            const int t = t0 + (e >> LOGPR), rho = e & (Pr - 1);
+            const double2 lh = k0 + t < Kc ? Ll[(size_t)(k0 + t) * Pr + rho] : make_double2(0.0, 0.0);
+            double2& v = buf[at<LOGPR>(t, rho)];
+            v = cmul(cconj(lh), v);
+        }
+        slab_sync<true>();
+        fft_slab<LOGPR, true>(buf, C, tw);
+        double2* Zq = Z + (size_t)item * Kc * Pr;
+        for (int e = lane; e < n_el; e += 64) {
+            const int t = t0 + (e >> LOGPR), rho = e & (Pr - 1);
+            if (k0 + t < Kc) Zq[(size_t)(k0 + t) * Pr + rho] = cconj(buf[at<LOGPR>(t, rho)]);
+        }
+        return;
+    }
     for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
         const uint32_t k = k0 + (idx >> logPr), rho = idx & (Pr - 1);
         buf[at<LOGPR>(idx >> logPr, rho)] = (k < Kc && rho < H) ? Sm[(size_t)k * H + rho] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_slab<LOGPR>(buf, C, tw);
-    double2* Ll = Lhat + (size_t)l * Kc * Pr;
     if (mode == 0) {
         for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
             const uint32_t k = k0 + (idx >> logPr);
@@ -340,10 +466,29 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
         if (kk > 0 && kk < half) buf[at<LOGPC>(t, Pc - kk)] = cadd(A, mul_mi(B));
     }
     __syncthreads();
-    fft_slab<LOGPC>(buf, T, tw);
-    // IFFT(Y) = conj(FFT(conj(Y))) / Pc: row A is the real part, row B minus the imaginary.
     Tout* Oq = O + (ooff ? ooff[q0 + item] : (size_t)(q0 + item) * (2 * H - 1) * SX);
     const int lane = threadIdx.x & 63;
+    if constexpr (LOGPC == 9) {
+        // T = 4: wave w transforms rows rho0 + 2w, +1 (fft512_regs) and writes them from
+        // registers: lane i holds circular columns c = i + 64 r.
+        const int t = threadIdx.x >> 6;
+        double2 u[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) u[r] = buf[at<9>(t, lane + 64 * r)];
+        fft512_regs(u, buf + at<9>(t, 0), tw);
+        const int oa = out_row(rho0 + 2 * t, H, Pr), ob = out_row(rho0 + 2 * t + 1, H, Pr);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int c = lane + 64 * r;
+            const int ox = c <= (int)W - 1 ? c + (int)W - 1 : (c >= 512 - (int)W + 1 ? c - (512 - (int)W + 1) : -1);
+            if (ox < 0) continue;
+            if (oa >= 0) Oq[(size_t)oa * SX + ox] = (Tout)(u[r].x * scale);
+            if (ob >= 0) Oq[(size_t)ob * SX + ox] = (Tout)(-u[r].y * scale);
+        }
+        return;
+    }
+    fft_slab<LOGPC>(buf, T, tw);
+    // IFFT(Y) = conj(FFT(conj(Y))) / Pc: row A is the real part, row B minus the imaginary.
     for (int rr = threadIdx.x >> 6; rr < rows2; rr += kFT / 64) {
         const int oy = out_row(rho0 + rr, H, Pr);
         if (oy < 0) continue;

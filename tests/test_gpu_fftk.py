@@ -67,6 +67,9 @@ CASES = [
     ("n_to_mn", 32, 32, 2, 6, "u01"),
     ("n_to_m", 64, 64, 4, 6, "normal"),
     ("n_to_m", 17, 23, 3, 5, "u11"),
+    ("one_to_many", 129, 256, 1, 3, "u11"),   # 512-point register path: rows and columns
+    ("n_to_m", 200, 140, 2, 2, "normal"),     # 512-point columns, 512-point rows, partial H
+    ("one_to_one", 256, 70, 1, 1, "u01"),     # 512-point columns only
 ]
 
 
