@@ -195,48 +195,62 @@ __device__ __forceinline__ void fft_slab(double2* buf, int T, const double2* __r
     if constexpr (WAVE && !WAVE_ONLY) __syncthreads();
 }
 
-// 512-point FFT owned by one wave, entered and left in registers: lane i holds x[i + 64 r]
-// (r = 0..7) on entry and X[i + 64 r] on return.  The three radix-8 passes exchange data
-// through the wave's LDS area wb (slots at<9>(0, j)) twice; the first pass reads, and the last
-// pass writes, registers only (its outputs i + 64 r are its inputs' positions).  So a kernel
-// that loads x[i + 64 r] from HBM and stores X[i + 64 r] to HBM needs no other LDS traffic.
-__device__ __forceinline__ void fft512_regs(double2 (&u)[8], double2* wb, const double2* __restrict__ tw) {
-    const int i = threadIdx.x & 63;
-    // pass 1 (p = 1): no twiddles; outputs 8 i + r
-    dft<8>(u);
+// Radix-8 passes of fft_regs from p = 2^LOGPP on: middle passes through LDS, the last one
+// (p = P/8, k = i) leaves its outputs i + (P/8) r in registers.
+template <int LOGP, int LOGPP>
+__device__ __forceinline__ void fft_regs_passes(double2 (&u)[8], double2* wb, int tl, int i,
+                                                const double2* __restrict__ tw) {
+    constexpr int Q = 1 << (LOGP - 3);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) wb[at<9>(0, 8 * i + r)] = u[r];
-    slab_sync<true>();
-    // pass 2 (p = 8): k = i mod 8; outputs (i - k) * 8 + k + 8 r
-    {
-        const int k = i & 7;
-#pragma unroll
-        for (int r = 0; r < 8; ++r) u[r] = wb[at<9>(0, i + 64 * r)];
-        const double2 w1 = tw[k << 6];
+    for (int r = 0; r < 8; ++r) u[r] = wb[at<LOGP>(tl, i + Q * r)];
+    const int k = i & ((1 << LOGPP) - 1);
+    if constexpr (LOGPP > 0) {
+        const double2 w1 = tw[k << (kLogMaxP - LOGPP - 3)];
         double2 w = w1;
 #pragma unroll
         for (int r = 1; r < 8; ++r) {
             u[r] = cmul(u[r], w);
             if (r + 1 < 8) w = cmul(w, w1);
         }
-        dft<8>(u);
-        slab_sync<true>();
-#pragma unroll
-        for (int r = 0; r < 8; ++r) wb[at<9>(0, ((i - k) << 3) + k + 8 * r)] = u[r];
-        slab_sync<true>();
-    }
-    // pass 3 (p = 64): k = i; outputs i + 64 r, kept in registers
-#pragma unroll
-    for (int r = 0; r < 8; ++r) u[r] = wb[at<9>(0, i + 64 * r)];
-    const double2 w1 = tw[i << 3];
-    double2 w = w1;
-#pragma unroll
-    for (int r = 1; r < 8; ++r) {
-        u[r] = cmul(u[r], w);
-        if (r + 1 < 8) w = cmul(w, w1);
     }
     dft<8>(u);
-    slab_sync<true>();  // the area may be written again right after
+    slab_sync<true>();
+    if constexpr (LOGPP + 3 < LOGP) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) wb[at<LOGP>(tl, ((i - k) << 3) + k + (r << LOGPP))] = u[r];
+        slab_sync<true>();
+        fft_regs_passes<LOGP, LOGPP + 3>(u, wb, tl, i, tw);
+    }
+}
+
+// Wave-owned FFTs entered and left in registers, for P = 2^LOGP of 16..512 points.  The wave
+// holds 512/P transforms; lane (tl, i) -- tl = lane / Q, i = lane mod Q, Q = P/8 -- holds
+// elements i + Q s (s = 0..7) of its transform tl on entry, and the transform's outputs at the
+// same positions on return.  The first pass (radix R0 = 8, 2 or 4 as log2 P mod 3 is 0, 1, 2;
+// 8/R0 butterflies per lane, no twiddles) reads registers only and the last radix-8 pass
+// writes registers only, so a kernel that moves those positions to and from HBM needs LDS
+// only for the exchanges between passes (one up to 64 points, two up to 512).  wb is the
+// wave's first transform slot; the wave's transform tl lives at wb + at<LOGP>(tl, .).
+template <int LOGP>
+__device__ __forceinline__ void fft_regs(double2 (&u)[8], double2* wb, const double2* __restrict__ tw) {
+    static_assert(LOGP >= 4 && LOGP <= 9, "register FFTs take 16..512 points");
+    constexpr int Q = 1 << (LOGP - 3);
+    constexpr int REM = LOGP % 3;
+    constexpr int R0 = REM == 1 ? 2 : (REM == 2 ? 4 : 8);
+    constexpr int LOGR0 = REM == 0 ? 3 : REM;
+    const int lane = threadIdx.x & 63, tl = lane >> (LOGP - 3), i = lane & (Q - 1);
+    // first pass (p = 1): butterfly b = i + Q m takes elements b + (P/R0) r = u[m + (8/R0) r]
+#pragma unroll
+    for (int m = 0; m < 8 / R0; ++m) {
+        double2 v[R0];
+#pragma unroll
+        for (int r = 0; r < R0; ++r) v[r] = u[m + (8 / R0) * r];
+        dft<R0>(v);
+#pragma unroll
+        for (int r = 0; r < R0; ++r) wb[at<LOGP>(tl, (i + Q * m) * R0 + r)] = v[r];
+    }
+    slab_sync<true>();
+    fft_regs_passes<LOGP, LOGR0>(u, wb, tl, i, tw);
 }
 
 // rows_fwd: block (matrix, group of 2T rows); matrices 0..nl-1 are the left ones (into SL),
@@ -256,21 +270,23 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
     const uint32_t r0 = g << (logT + 1);
     const Tin* Xm = (left ? XL : XR) + (moff ? moff[matg] : (size_t)mat * H * W);
     double2* S = left ? SL : SR;
-    if constexpr (LOGPC == 9) {
-        // T = 4: wave w transforms rows r0 + 2w, r0 + 2w + 1 in registers (fft512_regs) and
-        // leaves the spectrum in natural order in its LDS area for the block's split below.
-        const int lane = threadIdx.x & 63, t = threadIdx.x >> 6;
+    if constexpr (LOGPC <= 9) {
+        // T = 2048/Pc: the wave's transforms (row pairs) go through fft_regs from HBM; the
+        // spectra are left in natural order in LDS for the block's split below.
+        constexpr int Q = 1 << (LOGPC - 3);
+        const int lane = threadIdx.x & 63, tl = lane >> (LOGPC - 3), i = lane & (Q - 1);
+        const int t0 = (threadIdx.x >> 6) << (9 - LOGPC), t = t0 + tl;
         const uint32_t a = r0 + 2 * t, b = a + 1;
         double2 u[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const uint32_t c = lane + 64 * r;
+            const uint32_t c = i + Q * r;
             u[r] = make_double2((a < H && c < W) ? (double)Xm[(size_t)a * W + c] : 0.0,
                                 (b < H && c < W) ? (double)Xm[(size_t)b * W + c] : 0.0);
         }
-        fft512_regs(u, buf + at<9>(t, 0), tw);
+        fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), tw);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) buf[at<9>(t, lane + 64 * r)] = u[r];
+        for (int r = 0; r < 8; ++r) buf[at<LOGPC>(t, i + Q * r)] = u[r];
         __syncthreads();
     } else {
         for (int idx = threadIdx.x; idx < (T << logPc); idx += kFT) {
@@ -333,32 +349,38 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
     }
     const double2* Sm = S + (size_t)r * Kc * H;
     double2* Ll = Lhat + (size_t)l * Kc * Pr;
-    if constexpr (LOGPR == 9) {
-        // C = 4: wave w owns column k0 + w from HBM to HBM in registers (fft512_regs); the
-        // product with conj(Lhat) happens in registers between the two FFTs.
-        const int lane = threadIdx.x & 63, t = threadIdx.x >> 6;
-        const uint32_t k = k0 + t;
-        if (k >= Kc) return;
-        double2* wb = buf + at<9>(t, 0);
+    if constexpr (LOGPR <= 9) {
+        // C = 2048/Pr: the wave's columns go from HBM to HBM through fft_regs; the product with
+        // conj(Lhat) happens in registers between the two FFTs.
+        constexpr int Q = 1 << (LOGPR - 3);
+        const int lane = threadIdx.x & 63, tl = lane >> (LOGPR - 3), i = lane & (Q - 1);
+        const int t0 = (threadIdx.x >> 6) << (9 - LOGPR);
+        const uint32_t k = k0 + t0 + tl;
+        const bool ok = k < Kc;
+        double2* wb = buf + at<LOGPR>(t0, 0);
         double2 u[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const uint32_t rho = lane + 64 * r;
-            u[r] = rho < H ? Sm[(size_t)k * H + rho] : make_double2(0.0, 0.0);
+            const uint32_t rho = i + Q * r;
+            u[r] = ok && rho < H ? Sm[(size_t)k * H + rho] : make_double2(0.0, 0.0);
         }
-        fft512_regs(u, wb, tw);
-        const double2* lk = Ll + (size_t)k * 512;
+        fft_regs<LOGPR>(u, wb, tw);
+        double2* lk = Ll + (size_t)k * Pr;
         if (mode == 0) {
+            if (ok) {
 #pragma unroll
-            for (int r = 0; r < 8; ++r) Ll[(size_t)k * 512 + lane + 64 * r] = u[r];
+                for (int r = 0; r < 8; ++r) lk[i + Q * r] = u[r];
+            }
             return;
         }
 #pragma unroll
-        for (int r = 0; r < 8; ++r) u[r] = cmul(cconj(lk[lane + 64 * r]), u[r]);
-        fft512_regs(u, wb, tw);
-        double2* zk = Z + (size_t)item * Kc * Pr + (size_t)k * 512;
+        for (int r = 0; r < 8; ++r) u[r] = cmul(cconj(ok ? lk[i + Q * r] : make_double2(0.0, 0.0)), u[r]);
+        fft_regs<LOGPR>(u, wb, tw);
+        if (ok) {
+            double2* zk = Z + (size_t)item * Kc * Pr + (size_t)k * Pr;
 #pragma unroll
-        for (int r = 0; r < 8; ++r) zk[lane + 64 * r] = cconj(u[r]);
+            for (int r = 0; r < 8; ++r) zk[i + Q * r] = cconj(u[r]);
+        }
         return;
     }
     if constexpr (slab_of<LOGPR>::WAVE) {
@@ -425,8 +447,9 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
 }
 
 // Output row of circular row rho: dy = rho for rho < H, rho - Pr for rho > Pr - H; -1 for the
-// rows in between (not part of the (2H-1)-row result).
+// rows in between (not part of the (2H-1)-row result) and for rho >= Pr.
 __device__ __forceinline__ int out_row(uint32_t rho, uint32_t H, uint32_t Pr) {
+    if (rho >= Pr) return -1;
     if (rho < H) return (int)(rho + H - 1);
     if (rho > Pr - H) return (int)(rho - (Pr - H + 1));
     return -1;
@@ -460,27 +483,30 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
     // extended by A[Pc-k] = conj(A[k]), B[Pc-k] = conj(B[k]).
     for (int idx = threadIdx.x; idx < (int)Kc << logT; idx += kFT) {
         const int kk = idx >> logT, t = idx & (T - 1);
-        const double2* zc = Zq + (size_t)kk * Pr + rho0 + 2 * t;
-        const double2 A = zc[0], B = zc[1];
+        const uint32_t ra = rho0 + 2 * t;  // rows at and beyond Pr (a block taller than Pr) are zero
+        const double2* zc = Zq + (size_t)kk * Pr + ra;
+        const double2 A = ra < Pr ? zc[0] : make_double2(0.0, 0.0), B = ra + 1 < Pr ? zc[1] : make_double2(0.0, 0.0);
         buf[at<LOGPC>(t, kk)] = cadd(cconj(A), mul_mi(cconj(B)));
         if (kk > 0 && kk < half) buf[at<LOGPC>(t, Pc - kk)] = cadd(A, mul_mi(B));
     }
     __syncthreads();
     Tout* Oq = O + (ooff ? ooff[q0 + item] : (size_t)(q0 + item) * (2 * H - 1) * SX);
     const int lane = threadIdx.x & 63;
-    if constexpr (LOGPC == 9) {
-        // T = 4: wave w transforms rows rho0 + 2w, +1 (fft512_regs) and writes them from
-        // registers: lane i holds circular columns c = i + 64 r.
-        const int t = threadIdx.x >> 6;
+    if constexpr (LOGPC <= 9) {
+        // T = 2048/Pc: the wave's transforms (row pairs) go through fft_regs and are written
+        // from registers: lane (tl, i) holds circular columns c = i + Q r of rows rho0 + 2t, +1.
+        constexpr int Q = 1 << (LOGPC - 3);
+        const int tl = lane >> (LOGPC - 3), i = lane & (Q - 1);
+        const int t0 = (threadIdx.x >> 6) << (9 - LOGPC), t = t0 + tl;
         double2 u[8];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) u[r] = buf[at<9>(t, lane + 64 * r)];
-        fft512_regs(u, buf + at<9>(t, 0), tw);
+        for (int r = 0; r < 8; ++r) u[r] = buf[at<LOGPC>(t, i + Q * r)];
+        fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), tw);
         const int oa = out_row(rho0 + 2 * t, H, Pr), ob = out_row(rho0 + 2 * t + 1, H, Pr);
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const int c = lane + 64 * r;
-            const int ox = c <= (int)W - 1 ? c + (int)W - 1 : (c >= 512 - (int)W + 1 ? c - (512 - (int)W + 1) : -1);
+            const int c = i + Q * r;
+            const int ox = c <= (int)W - 1 ? c + (int)W - 1 : (c >= Pc - (int)W + 1 ? c - (Pc - (int)W + 1) : -1);
             if (ox < 0) continue;
             if (oa >= 0) Oq[(size_t)oa * SX + ox] = (Tout)(u[r].x * scale);
             if (ob >= 0) Oq[(size_t)ob * SX + ox] = (Tout)(-u[r].y * scale);
@@ -562,7 +588,9 @@ slab_shape shape_for(int logP, uint32_t want) {
     slab_shape v{};
     v.logS = std::max(11, logP);  // as slab_of<LOGP>
     v.wave = logP <= 9;
-    int lt = v.wave ? 2 : 0;
+    // wave-owned lengths use the full slab: the register FFTs give each lane 8 values of one
+    // transform, 512/P transforms per wave
+    int lt = v.wave ? v.logS - logP : 0;
     while (lt + logP < v.logS && (1u << lt) < want) ++lt;
     v.logT = lt;
     v.lds = (uint32_t)(((1u << lt) * ((1u << logP) + (1u << logP) / 8 + 1)) * sizeof(double2));
@@ -614,10 +642,10 @@ fftk_geom geom_of(uint32_t H, uint32_t W) {
     // rows_inv 2T output rows.
     g.v1 = shape_for(g.logPc, (H + 1) / 2);
     g.vc = shape_for(g.logPr, g.Kc);
-    g.v3 = shape_for(g.logPc, g.Pr / 2);  // 2T <= Pr
+    g.v3 = shape_for(g.logPc, g.Pr / 2);
     g.g1 = (H + (2u << g.v1.logT) - 1) >> (g.v1.logT + 1);
     g.cg = (g.Kc + (1u << g.vc.logT) - 1) >> g.vc.logT;
-    g.g3 = g.Pr >> (g.v3.logT + 1);
+    g.g3 = (g.Pr + (2u << g.v3.logT) - 1) >> (g.v3.logT + 1);  // a block may be taller than Pr
     g.s_bytes = sizeof(double2) * (size_t)g.Kc * H;
     g.col_bytes = sizeof(double2) * (size_t)g.Kc * g.Pr;
     return g;
