@@ -25,7 +25,7 @@ namespace xc {
 
 namespace {
 
-constexpr uint32_t kFeatures = 14;
+constexpr uint32_t kFeatures = 15;
 
 // ------------------------------------------------------------------ candidates
 std::string candidate_name(uint32_t i) {
@@ -243,7 +243,9 @@ xc_options candidate(uint32_t i) {
 // Features: log2 of rows, cols, n, m, results and MACs; log2 of the FFT points transformed
 // at the reference's padding (2H x 2W); log2 of the power-of-two padding's area over the
 // needed (2H-1)(2W-1); log2 of the largest prime factor of 2H and of 2W (how well the
-// reference padding factors); the family one-hot.
+// reference padding factors); the family one-hot; log2 of the own-FFT method's butterfly
+// work (row transforms of every input, two column transforms and one row transform per
+// result, each n log2 n at the power-of-two sizes).
 uint32_t workload_features(const xc_workload& w, float* f, uint32_t cap) {
     const int64_t k = num_results((int)w.family, w.n, w.m);
     if (w.family > XC_N_TO_M || k < 0 || w.rows == 0 || w.cols == 0)
@@ -284,6 +286,10 @@ uint32_t workload_features(const xc_workload& w, float* f, uint32_t cap) {
     f[8] = (float)std::log2((double)lpf(2 * w.rows));
     f[9] = (float)std::log2((double)lpf(2 * w.cols));
     for (uint32_t i = 0; i < 4; ++i) f[10 + i] = w.family == i ? 1.f : 0.f;
+    const double nin = w.family == XC_ONE_TO_ONE ? 2.0 : (w.family == XC_ONE_TO_MANY ? 1.0 + w.m : (double)w.n + w.m);
+    const double own = nin * (double)w.rows * pc * std::log2(pc) / 2 + (double)k * (pc / 2 + 1) * pr * std::log2(pr) * 2 +
+                       (double)k * pr * pc * std::log2(pc) / 2;
+    f[14] = (float)std::log2(own);
     return kFeatures;
 }
 

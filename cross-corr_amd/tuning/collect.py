@@ -2,7 +2,7 @@
 XC_PREC_AUTO on seeded random workloads, timed by the library's own exhaustive autotuner
 (xc_autotune_f32: HIP events on the stream, one warm-up then `reps` runs).
 
-    python cross-corr_amd/tuning/collect.py --count 300 --large 60 --seed 2024 --out cross-corr_amd/tuning/timings.csv
+    python cross-corr_amd/tuning/collect.py --count 300 --large 150 --seed 2024 --out cross-corr_amd/tuning/timings.csv
 
 Inputs are U[0,1) float32 from a seed per workload.  The `--large` workloads (more work than
 the definition-based methods can be timed on in a second) get times for the FFT methods only;
@@ -30,7 +30,7 @@ def main():
     ap.add_argument("--count", type=int, default=300)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--reps", type=int, default=3)
-    ap.add_argument("--large", type=int, default=60, help="extra large workloads, FFT methods only")
+    ap.add_argument("--large", type=int, default=150, help="extra large workloads, FFT methods only")
     ap.add_argument("--out", default=str(HERE / "timings.csv"))
     args = ap.parse_args()
     import torch

@@ -21,7 +21,7 @@ def main():
     ap.add_argument("--timings", default=str(HERE / "timings.csv"))
     ap.add_argument("--out", default=str(HERE / "cost_model.txt"))
     ap.add_argument("--holdout", type=int, default=5)
-    ap.add_argument("--epochs", type=int, default=4000)
+    ap.add_argument("--epochs", type=int, default=6000)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--weight-decay", type=float, default=1e-4)
     ap.add_argument("--hidden", type=int, default=32)
