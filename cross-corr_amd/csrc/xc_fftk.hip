@@ -195,23 +195,42 @@ __device__ __forceinline__ void fft_slab(double2* buf, int T, const double2* __r
     if constexpr (WAVE && !WAVE_ONLY) __syncthreads();
 }
 
+// Pass roots of fft_regs<LOGP> at lane position i: w[j] = exp(-2 pi i k / (8p)), k = i mod p,
+// for the j-th twiddled pass (p = 2^(LOGR0 + 3j)).  Kernels load them once, before their
+// first transform: a table read inside a pass would wait on memory after every LDS exchange.
+template <int LOGP>
+struct regs_roots {
+    static constexpr int REM = LOGP % 3;
+    static constexpr int LOGR0 = REM == 0 ? 3 : REM;  // log2 of the first (untwiddled) radix
+    static constexpr int N = (LOGP - LOGR0) / 3;     // twiddled radix-8 passes
+    double2 w[N];
+};
+template <int LOGP>
+__device__ __forceinline__ regs_roots<LOGP> load_roots(int i, const double2* __restrict__ tw) {
+    regs_roots<LOGP> rr;
+#pragma unroll
+    for (int j = 0; j < regs_roots<LOGP>::N; ++j) {
+        const int logpp = regs_roots<LOGP>::LOGR0 + 3 * j;
+        rr.w[j] = tw[(i & ((1 << logpp) - 1)) << (kLogMaxP - logpp - 3)];
+    }
+    return rr;
+}
+
 // Radix-8 passes of fft_regs from p = 2^LOGPP on: middle passes through LDS, the last one
 // (p = P/8, k = i) leaves its outputs i + (P/8) r in registers.
 template <int LOGP, int LOGPP>
 __device__ __forceinline__ void fft_regs_passes(double2 (&u)[8], double2* wb, int tl, int i,
-                                                const double2* __restrict__ tw) {
+                                                const regs_roots<LOGP>& rr) {
     constexpr int Q = 1 << (LOGP - 3);
 #pragma unroll
     for (int r = 0; r < 8; ++r) u[r] = wb[at<LOGP>(tl, i + Q * r)];
     const int k = i & ((1 << LOGPP) - 1);
-    if constexpr (LOGPP > 0) {
-        const double2 w1 = tw[k << (kLogMaxP - LOGPP - 3)];
-        double2 w = w1;
+    const double2 w1 = rr.w[(LOGPP - regs_roots<LOGP>::LOGR0) / 3];
+    double2 w = w1;
 #pragma unroll
-        for (int r = 1; r < 8; ++r) {
-            u[r] = cmul(u[r], w);
-            if (r + 1 < 8) w = cmul(w, w1);
-        }
+    for (int r = 1; r < 8; ++r) {
+        u[r] = cmul(u[r], w);
+        if (r + 1 < 8) w = cmul(w, w1);
     }
     dft<8>(u);
     slab_sync<true>();
@@ -219,7 +238,7 @@ __device__ __forceinline__ void fft_regs_passes(double2 (&u)[8], double2* wb, in
 #pragma unroll
         for (int r = 0; r < 8; ++r) wb[at<LOGP>(tl, ((i - k) << 3) + k + (r << LOGPP))] = u[r];
         slab_sync<true>();
-        fft_regs_passes<LOGP, LOGPP + 3>(u, wb, tl, i, tw);
+        fft_regs_passes<LOGP, LOGPP + 3>(u, wb, tl, i, rr);
     }
 }
 
@@ -230,14 +249,14 @@ __device__ __forceinline__ void fft_regs_passes(double2 (&u)[8], double2* wb, in
 // 8/R0 butterflies per lane, no twiddles) reads registers only and the last radix-8 pass
 // writes registers only, so a kernel that moves those positions to and from HBM needs LDS
 // only for the exchanges between passes (one up to 64 points, two up to 512).  wb is the
-// wave's first transform slot; the wave's transform tl lives at wb + at<LOGP>(tl, .).
+// wave's first transform slot; the wave's transform tl lives at wb + at<LOGP>(tl, .).  rr holds
+// the pass roots of this lane (load_roots<LOGP>(i, table)).
 template <int LOGP>
-__device__ __forceinline__ void fft_regs(double2 (&u)[8], double2* wb, const double2* __restrict__ tw) {
+__device__ __forceinline__ void fft_regs(double2 (&u)[8], double2* wb, const regs_roots<LOGP>& rr) {
     static_assert(LOGP >= 4 && LOGP <= 9, "register FFTs take 16..512 points");
     constexpr int Q = 1 << (LOGP - 3);
-    constexpr int REM = LOGP % 3;
-    constexpr int R0 = REM == 1 ? 2 : (REM == 2 ? 4 : 8);
-    constexpr int LOGR0 = REM == 0 ? 3 : REM;
+    constexpr int LOGR0 = regs_roots<LOGP>::LOGR0;
+    constexpr int R0 = 1 << LOGR0;
     const int lane = threadIdx.x & 63, tl = lane >> (LOGP - 3), i = lane & (Q - 1);
     // first pass (p = 1): butterfly b = i + Q m takes elements b + (P/R0) r = u[m + (8/R0) r]
 #pragma unroll
@@ -250,7 +269,7 @@ __device__ __forceinline__ void fft_regs(double2 (&u)[8], double2* wb, const dou
         for (int r = 0; r < R0; ++r) wb[at<LOGP>(tl, (i + Q * m) * R0 + r)] = v[r];
     }
     slab_sync<true>();
-    fft_regs_passes<LOGP, LOGR0>(u, wb, tl, i, tw);
+    fft_regs_passes<LOGP, LOGR0>(u, wb, tl, i, rr);
 }
 
 // rows_fwd: block (matrix, group of 2T rows); matrices 0..nl-1 are the left ones (into SL),
@@ -277,14 +296,27 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
         const int lane = threadIdx.x & 63, tl = lane >> (LOGPC - 3), i = lane & (Q - 1);
         const int t0 = (threadIdx.x >> 6) << (9 - LOGPC), t = t0 + tl;
         const uint32_t a = r0 + 2 * t, b = a + 1;
+        const regs_roots<LOGPC> rr = load_roots<LOGPC>(i, tw);
+        // Branch-free loads from clamped (in-bounds) addresses, all issued before the first
+        // value is used; out-of-range positions are zeroed afterwards.  (Guarded loads make
+        // the compiler wait on memory inside each guard.)
+        const bool oka = a < H, okb = b < H;
+        const Tin* xa = Xm + (oka ? (size_t)a * W : 0);
+        const Tin* xb = Xm + (okb ? (size_t)b * W : 0);
+        Tin va[8], vb[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint32_t c = i + Q * r, cc = c < W ? c : W - 1;
+            va[r] = xa[cc];
+            vb[r] = xb[cc];
+        }
         double2 u[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const uint32_t c = i + Q * r;
-            u[r] = make_double2((a < H && c < W) ? (double)Xm[(size_t)a * W + c] : 0.0,
-                                (b < H && c < W) ? (double)Xm[(size_t)b * W + c] : 0.0);
+            const bool in = i + Q * r < (int)W;
+            u[r] = make_double2(oka && in ? (double)va[r] : 0.0, okb && in ? (double)vb[r] : 0.0);
         }
-        fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), tw);
+        fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), rr);
 #pragma unroll
         for (int r = 0; r < 8; ++r) buf[at<LOGPC>(t, i + Q * r)] = u[r];
         __syncthreads();
@@ -358,14 +390,23 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
         const uint32_t k = k0 + t0 + tl;
         const bool ok = k < Kc;
         double2* wb = buf + at<LOGPR>(t0, 0);
-        double2 u[8];
+        const regs_roots<LOGPR> rr = load_roots<LOGPR>(i, tw);
+        double2 u[8], lh[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const uint32_t rho = i + Q * r;
-            u[r] = ok && rho < H ? Sm[(size_t)k * H + rho] : make_double2(0.0, 0.0);
+            u[r] = make_double2(0.0, 0.0);
+            if (ok && rho < H) u[r] = Sm[(size_t)k * H + rho];
         }
-        fft_regs<LOGPR>(u, wb, tw);
         double2* lk = Ll + (size_t)k * Pr;
+        // The left spectrum is loaded together with the column (its latency overlaps the
+        // first transform's).
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            lh[r] = make_double2(0.0, 0.0);
+            if (mode == 1 && ok) lh[r] = lk[i + Q * r];
+        }
+        fft_regs<LOGPR>(u, wb, rr);
         if (mode == 0) {
             if (ok) {
 #pragma unroll
@@ -374,8 +415,8 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
             return;
         }
 #pragma unroll
-        for (int r = 0; r < 8; ++r) u[r] = cmul(cconj(ok ? lk[i + Q * r] : make_double2(0.0, 0.0)), u[r]);
-        fft_regs<LOGPR>(u, wb, tw);
+        for (int r = 0; r < 8; ++r) u[r] = cmul(cconj(lh[r]), u[r]);
+        fft_regs<LOGPR>(u, wb, rr);
         if (ok) {
             double2* zk = Z + (size_t)item * Kc * Pr + (size_t)k * Pr;
 #pragma unroll
@@ -478,9 +519,37 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
     // Groups wholly between the two halves of the result have nothing to write.
     if (out_row(rho0, H, Pr) < 0 && out_row(rho0 + rows2 - 1, H, Pr) < 0 && rho0 >= H) return;
     const double2* Zq = Z + (size_t)item * Kc * Pr;
+    const int lane = threadIdx.x & 63;
+    regs_roots<LOGPC> rr{};
+    if constexpr (LOGPC <= 9) rr = load_roots<LOGPC>(lane & ((1 << (LOGPC - 3)) - 1), tw);
     // One thread per (column kk, transform t) loads the two rows' values A[kk], B[kk] (32
     // adjacent bytes) and writes both slots kk and Pc-kk of conj(Y), Y[k] = A[k] + i B[k]
     // extended by A[Pc-k] = conj(A[k]), B[Pc-k] = conj(B[k]).
+    if constexpr (LOGPC <= 9) {
+        // T = 2048/Pc, so (Kc << logT) = 1024 + T <= 5 * kFT: all loads first, then the writes.
+        constexpr int NQ = (1024 + (2048 >> LOGPC) + kFT - 1) / kFT;
+        const int items = (int)Kc << logT;
+        double2 A[NQ], B[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int idx = threadIdx.x + kFT * q;
+            const int kk = idx >> logT, t = idx & (T - 1);
+            const uint32_t ra = rho0 + 2 * t;
+            const double2* zc = Zq + (size_t)kk * Pr + ra;
+            A[q] = make_double2(0.0, 0.0);
+            B[q] = make_double2(0.0, 0.0);
+            if (idx < items && ra < Pr) A[q] = zc[0];
+            if (idx < items && ra + 1 < Pr) B[q] = zc[1];
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int idx = threadIdx.x + kFT * q;
+            if (idx >= items) continue;
+            const int kk = idx >> logT, t = idx & (T - 1);
+            buf[at<LOGPC>(t, kk)] = cadd(cconj(A[q]), mul_mi(cconj(B[q])));
+            if (kk > 0 && kk < half) buf[at<LOGPC>(t, Pc - kk)] = cadd(A[q], mul_mi(B[q]));
+        }
+    } else
     for (int idx = threadIdx.x; idx < (int)Kc << logT; idx += kFT) {
         const int kk = idx >> logT, t = idx & (T - 1);
         const uint32_t ra = rho0 + 2 * t;  // rows at and beyond Pr (a block taller than Pr) are zero
@@ -491,7 +560,6 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
     }
     __syncthreads();
     Tout* Oq = O + (ooff ? ooff[q0 + item] : (size_t)(q0 + item) * (2 * H - 1) * SX);
-    const int lane = threadIdx.x & 63;
     if constexpr (LOGPC <= 9) {
         // T = 2048/Pc: the wave's transforms (row pairs) go through fft_regs and are written
         // from registers: lane (tl, i) holds circular columns c = i + Q r of rows rho0 + 2t, +1.
@@ -501,7 +569,7 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
         double2 u[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) u[r] = buf[at<LOGPC>(t, i + Q * r)];
-        fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), tw);
+        fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), rr);
         const int oa = out_row(rho0 + 2 * t, H, Pr), ob = out_row(rho0 + 2 * t + 1, H, Pr);
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
