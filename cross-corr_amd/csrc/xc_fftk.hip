@@ -18,9 +18,11 @@
 //              rows of Z are extended, inverse transformed, scaled by 1/(Pr*Pc) and written
 //              straight into the cropped (2H-1) x (2W-1) result (quadrant swap by index).
 //
-// All arithmetic is FP64.  Each block transforms a slab of up to 4096 complex values in LDS
-// with radix-8/4/2 Stockham passes (register butterflies, padded LDS index against bank
-// conflicts); twiddles come from a 4096-entry table.
+// All arithmetic is FP64, in radix-8/4/2 Stockham passes.  Transforms of up to 512 points
+// live in registers (one wave owns each, LDS only for the exchanges between passes);
+// longer ones are transformed by a block in an LDS slab of up to 4096 values (padded LDS
+// index against bank conflicts).  Twiddles come from a 4096-entry table, read once per kernel
+// into registers; every HBM load of a kernel is issued before its first use.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -47,12 +49,11 @@ __device__ __forceinline__ int at(int t, int j) {
     return t * S + j + (j >> 3);
 }
 
-// Slab (values per block) and wave ownership of a transform length: 2^11 values, or one
-// transform of 4096; transforms of up to 512 points are owned by single waves (T >= 4).
+// Slab (values per block) of a transform length: 2^11 values, or one transform of 4096.
+// Lengths of up to 512 points are transformed in registers (fft_regs), longer ones in slabs.
 template <int LOGP>
 struct slab_of {
     static constexpr int LOGS = LOGP <= 11 ? 11 : LOGP;
-    static constexpr bool WAVE = LOGP <= 9;
 };
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -103,53 +104,77 @@ __device__ __forceinline__ void dft(double2 (&u)[R]) {
     }
 }
 
-// Barrier between the read and write phases of a pass.  In WAVE mode every transform of the
-// slab is owned by one wave for all passes, so ordering that wave's own LDS accesses is enough.
-template <bool WAVE>
-__device__ __forceinline__ void slab_sync() {
-    if constexpr (WAVE) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    } else {
-        __syncthreads();
+// Barrier between the LDS phases of a wave-owned transform: every value a lane reads was
+// written by its own wave, so ordering the wave's LDS accesses is enough.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// First radix of a length-2^LOGP transform (8, 2 or 4 as LOGP mod 3 is 0, 1, 2; it takes no
+// twiddles) and the number of twiddled radix-8 passes after it.
+template <int LOGP>
+struct radix_plan {
+    static constexpr int REM = LOGP % 3;
+    static constexpr int LOGR0 = REM == 0 ? 3 : REM;
+    static constexpr int NTW = (LOGP - LOGR0) / 3;
+};
+
+// Pass roots of the slab transforms of length 2^LOGP (1024..4096): w[j][q] is the root of the
+// j-th twiddled radix-8 pass (p = 2^(LOGR0 + 3j)) for this thread's q-th butterfly, b =
+// threadIdx + kFT q.  Kernels load them once, before their first transform: a table read
+// inside a pass would wait on memory after every barrier.
+template <int LOGP>
+struct slab_roots {
+    static constexpr int NB8 = (1 << slab_of<LOGP>::LOGS) / 8 / kFT;  // radix-8 butterflies per thread
+    double2 w[radix_plan<LOGP>::NTW][NB8];
+};
+template <int LOGP>
+__device__ __forceinline__ slab_roots<LOGP> load_slab_roots(const double2* __restrict__ tw) {
+    slab_roots<LOGP> sr;
+#pragma unroll
+    for (int j = 0; j < radix_plan<LOGP>::NTW; ++j) {
+        const int logpp = radix_plan<LOGP>::LOGR0 + 3 * j;
+#pragma unroll
+        for (int q = 0; q < slab_roots<LOGP>::NB8; ++q) {
+            const int i = ((int)threadIdx.x + kFT * q) & ((1 << (LOGP - 3)) - 1);
+            sr.w[j][q] = tw[(i & ((1 << logpp) - 1)) << (kLogMaxP - logpp - 3)];
+        }
     }
+    return sr;
 }
 
 // One Stockham radix-R pass over T transforms of length P = 2^LOGP in buf (slots `at`),
 // T * P <= 2^LOGS; 2^LOGPP is the product of the radices already applied.  Butterfly i of a
 // transform reads elements i + r*P/R and writes (i - k)*R + k + r*p, k = i mod p, after
-// multiplying input r by exp(-2 pi i r k / (p R)).  Butterflies go to threads block-strided,
-// or (WAVE, T a multiple of 4) as one contiguous range of whole transforms per wave.
-template <int R, int LOGR, int LOGP, int LOGPP, int LOGS, bool WAVE>
-__device__ __forceinline__ void fft_pass(double2* buf, int T, const double2* __restrict__ tw) {
-    constexpr int MAXB = (1 << LOGS) / R / kFT;
+// multiplying input r by exp(-2 pi i r k / (p R)).  Butterflies go to threads block-strided.
+template <int R, int LOGR, int LOGP, int LOGPP>
+__device__ __forceinline__ void fft_pass(double2* buf, int T, const slab_roots<LOGP>& sr) {
+    constexpr int MAXB = (1 << slab_of<LOGP>::LOGS) / R / kFT;
     static_assert(MAXB >= 1, "slab too small for the radix");
     constexpr int LOGPR = LOGP - LOGR;
     const int nb = T << LOGPR;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int per_wave = nb >> 2;
     double2 u[MAXB][R];
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
-        const bool ok = WAVE ? lane + 64 * q < per_wave : (int)threadIdx.x + kFT * q < nb;
-        const int b = WAVE ? wave * per_wave + lane + 64 * q : (int)threadIdx.x + kFT * q;
-        if (ok) {
+        const int b = (int)threadIdx.x + kFT * q;
+        if (b < nb) {
             const int t = b >> LOGPR, i = b & ((1 << LOGPR) - 1);
 #pragma unroll
             for (int r = 0; r < R; ++r) u[q][r] = buf[at<LOGP>(t, i + (r << LOGPR))];
         }
     }
-    slab_sync<WAVE>();
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < MAXB; ++q) {
-        const bool ok = WAVE ? lane + 64 * q < per_wave : (int)threadIdx.x + kFT * q < nb;
-        const int b = WAVE ? wave * per_wave + lane + 64 * q : (int)threadIdx.x + kFT * q;
-        if (ok) {
+        const int b = (int)threadIdx.x + kFT * q;
+        if (b < nb) {
             const int t = b >> LOGPR, i = b & ((1 << LOGPR) - 1);
             const int k = i & ((1 << LOGPP) - 1);
             if constexpr (LOGPP > 0) {
-                const double2 w1 = tw[k << (kLogMaxP - LOGPP - LOGR)];
+                static_assert(R == 8, "only the first pass has another radix");
+                const double2 w1 = sr.w[(LOGPP - radix_plan<LOGP>::LOGR0) / 3][q];
                 double2 w = w1;
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
@@ -163,36 +188,25 @@ __device__ __forceinline__ void fft_pass(double2* buf, int T, const double2* __r
             for (int r = 0; r < R; ++r) buf[at<LOGP>(t, j + (r << LOGPP))] = u[q][r];
         }
     }
-    slab_sync<WAVE>();
+    __syncthreads();
 }
 
 template <int LOGP, int LOGPP>
-__device__ __forceinline__ void radix8_passes(double2* buf, int T, const double2* __restrict__ tw) {
+__device__ __forceinline__ void radix8_passes(double2* buf, int T, const slab_roots<LOGP>& sr) {
     if constexpr (LOGPP < LOGP) {
-        fft_pass<8, 3, LOGP, LOGPP, slab_of<LOGP>::LOGS, slab_of<LOGP>::WAVE>(buf, T, tw);
-        radix8_passes<LOGP, LOGPP + 3>(buf, T, tw);
+        fft_pass<8, 3, LOGP, LOGPP>(buf, T, sr);
+        radix8_passes<LOGP, LOGPP + 3>(buf, T, sr);
     }
 }
 
-// Forward FFT of T transforms of length 2^LOGP in buf; natural order.  The caller has
-// synchronised the block before (or, with WAVE_ONLY, the wave: every transform's data was
-// written by the wave that owns it); the block (WAVE_ONLY: the wave) is synchronised on return.
-template <int LOGP, bool WAVE_ONLY = false>
-__device__ __forceinline__ void fft_slab(double2* buf, int T, const double2* __restrict__ tw) {
-    constexpr int LOGS = slab_of<LOGP>::LOGS;
-    constexpr bool WAVE = slab_of<LOGP>::WAVE;
-    constexpr int rem = LOGP % 3;
-    if constexpr (rem == 1) {
-        fft_pass<2, 1, LOGP, 0, LOGS, WAVE>(buf, T, tw);
-        radix8_passes<LOGP, 1>(buf, T, tw);
-    } else if constexpr (rem == 2) {
-        fft_pass<4, 2, LOGP, 0, LOGS, WAVE>(buf, T, tw);
-        radix8_passes<LOGP, 2>(buf, T, tw);
-    } else {
-        radix8_passes<LOGP, 0>(buf, T, tw);
-    }
-    static_assert(!WAVE_ONLY || WAVE, "wave-only synchronisation needs wave-owned transforms");
-    if constexpr (WAVE && !WAVE_ONLY) __syncthreads();
+// Forward FFT of T transforms of length 2^LOGP (1024..4096) in buf; natural order.  The caller
+// has synchronised the block before; the block is synchronised on return.
+template <int LOGP>
+__device__ __forceinline__ void fft_slab(double2* buf, int T, const slab_roots<LOGP>& sr) {
+    static_assert(LOGP >= 10, "shorter transforms run in registers");
+    constexpr int LOGR0 = radix_plan<LOGP>::LOGR0;
+    fft_pass<1 << LOGR0, LOGR0, LOGP, 0>(buf, T, sr);
+    radix8_passes<LOGP, LOGR0>(buf, T, sr);
 }
 
 // Pass roots of fft_regs<LOGP> at lane position i: w[j] = exp(-2 pi i k / (8p)), k = i mod p,
@@ -233,11 +247,11 @@ __device__ __forceinline__ void fft_regs_passes(double2 (&u)[8], double2* wb, in
         if (r + 1 < 8) w = cmul(w, w1);
     }
     dft<8>(u);
-    slab_sync<true>();
+    wave_sync();
     if constexpr (LOGPP + 3 < LOGP) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) wb[at<LOGP>(tl, ((i - k) << 3) + k + (r << LOGPP))] = u[r];
-        slab_sync<true>();
+        wave_sync();
         fft_regs_passes<LOGP, LOGPP + 3>(u, wb, tl, i, rr);
     }
 }
@@ -268,18 +282,19 @@ __device__ __forceinline__ void fft_regs(double2 (&u)[8], double2* wb, const reg
 #pragma unroll
         for (int r = 0; r < R0; ++r) wb[at<LOGP>(tl, (i + Q * m) * R0 + r)] = v[r];
     }
-    slab_sync<true>();
+    wave_sync();
     fft_regs_passes<LOGP, LOGR0>(u, wb, tl, i, rr);
 }
 
 // rows_fwd: block (matrix, group of 2T rows); matrices 0..nl-1 are the left ones (into SL),
-// the rest the right ones (into SR).  Matrix i starts at element moff[i] of its input when
-// the offset table is given (uneven batches), else at i * H * W.
+// the rest the right ones (into SR).  Matrix i starts at element moff[i]
+// of its input when the offset table is given (uneven batches), else at i * H * W.
 template <typename Tin, int LOGPC>
 __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ XL, const Tin* __restrict__ XR,
                                                        const uint64_t* __restrict__ moff, uint32_t nl, uint32_t H,
-                                                       uint32_t W, int logT, uint32_t groups, double2* __restrict__ SL,
-                                                       double2* __restrict__ SR, const double2* __restrict__ tw) {
+                                                       uint32_t W, int logT, uint32_t groups,
+                                                       double2* __restrict__ SL, double2* __restrict__ SR,
+                                                       const double2* __restrict__ tw) {
     constexpr int logPc = LOGPC;
     extern __shared__ double2 buf[];
     const uint32_t matg = blockIdx.x / groups, g = blockIdx.x - matg * groups;
@@ -321,15 +336,32 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
         for (int r = 0; r < 8; ++r) buf[at<LOGPC>(t, i + Q * r)] = u[r];
         __syncthreads();
     } else {
-        for (int idx = threadIdx.x; idx < (T << logPc); idx += kFT) {
-            const int t = idx >> logPc, c = idx & (Pc - 1);
+        // Slab: T = 2^LOGS / Pc transforms at most; element idx of the slab is column c of
+        // transform t.  Branch-free clamped loads first, then the LDS writes.
+        constexpr int NI = (1 << slab_of<LOGPC>::LOGS) / kFT;
+        const slab_roots<LOGPC> sr = load_slab_roots<LOGPC>(tw);
+        const int n_el = T << logPc;
+        Tin va[NI], vb[NI];
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            const int idx = threadIdx.x + kFT * q;
+            const uint32_t t = idx >> logPc, c = idx & (Pc - 1);
             const uint32_t a = r0 + 2 * t, b = a + 1;
-            const double re = (a < H && (uint32_t)c < W) ? (double)Xm[(size_t)a * W + c] : 0.0;
-            const double im = (b < H && (uint32_t)c < W) ? (double)Xm[(size_t)b * W + c] : 0.0;
-            buf[at<LOGPC>(t, c)] = make_double2(re, im);
+            const size_t cc = c < W ? c : W - 1;
+            va[q] = Xm[(size_t)(a < H ? a : H - 1) * W + cc];
+            vb[q] = Xm[(size_t)(b < H ? b : H - 1) * W + cc];
+        }
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            const int idx = threadIdx.x + kFT * q;
+            if (idx >= n_el) continue;
+            const uint32_t t = idx >> logPc, c = idx & (Pc - 1);
+            const uint32_t a = r0 + 2 * t, b = a + 1;
+            const bool in = c < W;
+            buf[at<LOGPC>(t, c)] = make_double2(a < H && in ? (double)va[q] : 0.0, b < H && in ? (double)vb[q] : 0.0);
         }
         __syncthreads();
-        fft_slab<LOGPC>(buf, T, tw);
+        fft_slab<LOGPC>(buf, T, sr);
     }
     const uint32_t Kc = (uint32_t)(Pc / 2 + 1);
     const int rows2 = 2 << logT;
@@ -423,67 +455,54 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
             for (int r = 0; r < 8; ++r) zk[i + Q * r] = cconj(u[r]);
         }
         return;
-    }
-    if constexpr (slab_of<LOGPR>::WAVE) {
-        // Wave-local: wave w owns columns [w*C/4, (w+1)*C/4) through every phase, so no block
-        // barrier is needed anywhere.
-        const int lane = threadIdx.x & 63, tpw = C >> 2, t0 = (threadIdx.x >> 6) * tpw;
-        const int n_el = tpw << LOGPR;
-        for (int e = lane; e < n_el; e += 64) {
-            const int t = t0 + (e >> LOGPR), rho = e & (Pr - 1);
-            const uint32_t k = k0 + t;
-            buf[at<LOGPR>(t, rho)] = (k < Kc && (uint32_t)rho < H) ? Sm[(size_t)k * H + rho] : make_double2(0.0, 0.0);
-        }
-        slab_sync<true>();
-        fft_slab<LOGPR, true>(buf, C, tw);
-        if (mode == 0) {
-            for (int e = lane; e < n_el; e += 64) {
-                const int t = t0 + (e >> LOGPR), rho = e & (Pr - 1);
-                if (k0 + t < Kc) Ll[(size_t)(k0 + t) * Pr + rho] = buf[at<LOGPR>(t, rho)];
-            }
-            return;
-        }
-        for (int e = lane; e < n_el; e += 64) {
-            const int t = t0 + (e >> LOGPR), rho = e & (Pr - 1);
-            const double2 lh = k0 + t < Kc ? Ll[(size_t)(k0 + t) * Pr + rho] : make_double2(0.0, 0.0);
-            double2& v = buf[at<LOGPR>(t, rho)];
-            v = cmul(cconj(lh), v);
-        }
-        slab_sync<true>();
-        fft_slab<LOGPR, true>(buf, C, tw);
-        double2* Zq = Z + (size_t)item * Kc * Pr;
-        for (int e = lane; e < n_el; e += 64) {
-            const int t = t0 + (e >> LOGPR), rho = e & (Pr - 1);
-            if (k0 + t < Kc) Zq[(size_t)(k0 + t) * Pr + rho] = cconj(buf[at<LOGPR>(t, rho)]);
-        }
-        return;
-    }
-    for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
+    } else {
+    // Slab: C = 2^LOGS / Pr columns; element idx of the slab is row rho of column k0 + t.
+    // All loads (the column and, for pairs, the left spectrum) are issued before the first
+    // transform, from clamped in-bounds addresses.
+    constexpr int NI = (1 << slab_of<LOGPR>::LOGS) / kFT;
+    const slab_roots<LOGPR> sr = load_slab_roots<LOGPR>(tw);
+    double2 xs[NI], lh[NI];
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int idx = threadIdx.x + kFT * q;
         const uint32_t k = k0 + (idx >> logPr), rho = idx & (Pr - 1);
-        buf[at<LOGPR>(idx >> logPr, rho)] = (k < Kc && rho < H) ? Sm[(size_t)k * H + rho] : make_double2(0.0, 0.0);
+        const uint32_t kk = k < Kc ? k : Kc - 1;
+        xs[q] = Sm[(size_t)kk * H + (rho < H ? rho : H - 1)];
+        lh[q] = mode == 1 ? Ll[(size_t)kk * Pr + rho] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int idx = threadIdx.x + kFT * q;
+        const uint32_t k = k0 + (idx >> logPr), rho = idx & (Pr - 1);
+        buf[at<LOGPR>(idx >> logPr, rho)] = (k < Kc && rho < H) ? xs[q] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_slab<LOGPR>(buf, C, tw);
+    fft_slab<LOGPR>(buf, C, sr);
     if (mode == 0) {
-        for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
+#pragma unroll
+        for (int q = 0; q < NI; ++q) {
+            const int idx = threadIdx.x + kFT * q;
             const uint32_t k = k0 + (idx >> logPr);
             if (k < Kc) Ll[(size_t)k * Pr + (idx & (Pr - 1))] = buf[at<LOGPR>(idx >> logPr, idx & (Pr - 1))];
         }
         return;
     }
     // conj(Lhat * conj(X)) = conj(Lhat) * X; its forward FFT, conjugated, is Pr * IFFT.
-    for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
-        const uint32_t k = k0 + (idx >> logPr);
-        const double2 lh = k < Kc ? Ll[(size_t)k * Pr + (idx & (Pr - 1))] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int idx = threadIdx.x + kFT * q;
         double2& v = buf[at<LOGPR>(idx >> logPr, idx & (Pr - 1))];
-        v = cmul(cconj(lh), v);
+        v = k0 + (idx >> logPr) < Kc ? cmul(cconj(lh[q]), v) : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    fft_slab<LOGPR>(buf, C, tw);
+    fft_slab<LOGPR>(buf, C, sr);
     double2* Zq = Z + (size_t)item * Kc * Pr;
-    for (int idx = threadIdx.x; idx < (C << logPr); idx += kFT) {
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int idx = threadIdx.x + kFT * q;
         const uint32_t k = k0 + (idx >> logPr);
         if (k < Kc) Zq[(size_t)k * Pr + (idx & (Pr - 1))] = cconj(buf[at<LOGPR>(idx >> logPr, idx & (Pr - 1))]);
+    }
     }
 }
 
@@ -521,13 +540,19 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
     const double2* Zq = Z + (size_t)item * Kc * Pr;
     const int lane = threadIdx.x & 63;
     regs_roots<LOGPC> rr{};
-    if constexpr (LOGPC <= 9) rr = load_roots<LOGPC>(lane & ((1 << (LOGPC - 3)) - 1), tw);
+    slab_roots<LOGPC> sr{};
+    if constexpr (LOGPC <= 9)
+        rr = load_roots<LOGPC>(lane & ((1 << (LOGPC - 3)) - 1), tw);
+    else
+        sr = load_slab_roots<LOGPC>(tw);
     // One thread per (column kk, transform t) loads the two rows' values A[kk], B[kk] (32
     // adjacent bytes) and writes both slots kk and Pc-kk of conj(Y), Y[k] = A[k] + i B[k]
     // extended by A[Pc-k] = conj(A[k]), B[Pc-k] = conj(B[k]).
-    if constexpr (LOGPC <= 9) {
-        // T = 2048/Pc, so (Kc << logT) = 1024 + T <= 5 * kFT: all loads first, then the writes.
-        constexpr int NQ = (1024 + (2048 >> LOGPC) + kFT - 1) / kFT;
+    {
+        // At most TMAX = 2^LOGS / Pc transforms, so (Kc << logT) <= (Pc/2 + 1) * TMAX items: all
+        // loads first, then the writes.
+        constexpr int TMAX = (1 << slab_of<LOGPC>::LOGS) >> LOGPC;
+        constexpr int NQ = (((1 << (LOGPC - 1)) + 1) * TMAX + kFT - 1) / kFT;
         const int items = (int)Kc << logT;
         double2 A[NQ], B[NQ];
 #pragma unroll
@@ -549,14 +574,6 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
             buf[at<LOGPC>(t, kk)] = cadd(cconj(A[q]), mul_mi(cconj(B[q])));
             if (kk > 0 && kk < half) buf[at<LOGPC>(t, Pc - kk)] = cadd(A[q], mul_mi(B[q]));
         }
-    } else
-    for (int idx = threadIdx.x; idx < (int)Kc << logT; idx += kFT) {
-        const int kk = idx >> logT, t = idx & (T - 1);
-        const uint32_t ra = rho0 + 2 * t;  // rows at and beyond Pr (a block taller than Pr) are zero
-        const double2* zc = Zq + (size_t)kk * Pr + ra;
-        const double2 A = ra < Pr ? zc[0] : make_double2(0.0, 0.0), B = ra + 1 < Pr ? zc[1] : make_double2(0.0, 0.0);
-        buf[at<LOGPC>(t, kk)] = cadd(cconj(A), mul_mi(cconj(B)));
-        if (kk > 0 && kk < half) buf[at<LOGPC>(t, Pc - kk)] = cadd(A, mul_mi(B));
     }
     __syncthreads();
     Tout* Oq = O + (ooff ? ooff[q0 + item] : (size_t)(q0 + item) * (2 * H - 1) * SX);
@@ -580,8 +597,8 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
             if (ob >= 0) Oq[(size_t)ob * SX + ox] = (Tout)(-u[r].y * scale);
         }
         return;
-    }
-    fft_slab<LOGPC>(buf, T, tw);
+    } else {
+    fft_slab<LOGPC>(buf, T, sr);
     // IFFT(Y) = conj(FFT(conj(Y))) / Pc: row A is the real part, row B minus the imaginary.
     for (int rr = threadIdx.x >> 6; rr < rows2; rr += kFT / 64) {
         const int oy = out_row(rho0 + rr, H, Pr);
@@ -593,6 +610,7 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
             const double2 f = buf[at<LOGPC>(t, c)];
             orow[ox] = (Tout)(((rr & 1) ? -f.y : f.x) * scale);
         }
+    }
     }
 }
 
