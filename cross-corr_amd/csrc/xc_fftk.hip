@@ -297,7 +297,10 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
                                                        const double2* __restrict__ tw) {
     constexpr int logPc = LOGPC;
     extern __shared__ double2 buf[];
-    const uint32_t matg = blockIdx.x / groups, g = blockIdx.x - matg * groups;
+    // XCD-aware order: the row groups of one matrix run on one XCD, so the partial lines of S
+    // that neighbouring groups write (2T rows of a column) merge in that XCD's L2.
+    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t matg = bid / groups, g = bid - matg * groups;
     const bool left = matg < nl;
     const uint32_t mat = left ? matg : matg - nl;
     const int Pc = 1 << logPc, T = 1 << logT;
@@ -527,7 +530,10 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
                                                        const double2* __restrict__ tw) {
     constexpr int logPc = LOGPC;
     extern __shared__ double2 buf[];
-    const uint32_t item = blockIdx.x / groups, g = blockIdx.x - item * groups;
+    // XCD-aware order: the row groups of one pair run on one XCD, so the lines of Z that
+    // neighbouring groups read in pieces (2T rows of a column) are fetched into its L2 once.
+    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t item = bid / groups, g = bid - item * groups;
     const int Pc = 1 << logPc, T = 1 << logT;
     const uint32_t Pr = 1u << logPr;
     const int half = Pc / 2;
