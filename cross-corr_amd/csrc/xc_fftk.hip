@@ -209,6 +209,22 @@ __device__ __forceinline__ void fft_slab(double2* buf, int T, const slab_roots<L
     radix8_passes<LOGP, LOGR0>(buf, T, sr);
 }
 
+// Transforms of up to 2048 points run in registers: P/8 threads hold one transform, 8 values
+// each.  Up to 512 points a transform lives in one wave (BLOCK false: the wave holds 512/P of
+// them); 1024 and 2048 points span two and four waves, so a 256-thread block holds 2048/P.
+template <int LOGP>
+struct regs_of {
+    static constexpr bool ON = LOGP <= 11;
+    static constexpr bool BLOCK = LOGP > 9;
+};
+template <bool BLOCK>
+__device__ __forceinline__ void regs_sync() {
+    if constexpr (BLOCK)
+        __syncthreads();
+    else
+        wave_sync();
+}
+
 // Pass roots of fft_regs<LOGP> at lane position i: w[j] = exp(-2 pi i k / (8p)), k = i mod p,
 // for the j-th twiddled pass (p = 2^(LOGR0 + 3j)).  Kernels load them once, before their
 // first transform: a table read inside a pass would wait on memory after every LDS exchange.
@@ -247,31 +263,33 @@ __device__ __forceinline__ void fft_regs_passes(double2 (&u)[8], double2* wb, in
         if (r + 1 < 8) w = cmul(w, w1);
     }
     dft<8>(u);
-    wave_sync();
+    regs_sync<regs_of<LOGP>::BLOCK>();
     if constexpr (LOGPP + 3 < LOGP) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) wb[at<LOGP>(tl, ((i - k) << 3) + k + (r << LOGPP))] = u[r];
-        wave_sync();
+        regs_sync<regs_of<LOGP>::BLOCK>();
         fft_regs_passes<LOGP, LOGPP + 3>(u, wb, tl, i, rr);
     }
 }
 
-// Wave-owned FFTs entered and left in registers, for P = 2^LOGP of 16..512 points.  The wave
-// holds 512/P transforms; lane (tl, i) -- tl = lane / Q, i = lane mod Q, Q = P/8 -- holds
-// elements i + Q s (s = 0..7) of its transform tl on entry, and the transform's outputs at the
-// same positions on return.  The first pass (radix R0 = 8, 2 or 4 as log2 P mod 3 is 0, 1, 2;
-// 8/R0 butterflies per lane, no twiddles) reads registers only and the last radix-8 pass
-// writes registers only, so a kernel that moves those positions to and from HBM needs LDS
-// only for the exchanges between passes (one up to 64 points, two up to 512).  wb is the
-// wave's first transform slot; the wave's transform tl lives at wb + at<LOGP>(tl, .).  rr holds
-// the pass roots of this lane (load_roots<LOGP>(i, table)).
+// FFTs entered and left in registers, for P = 2^LOGP of 16..2048 points (regs_of).  Thread
+// (tl, i) of the owning wave (up to 512 points) or block (1024, 2048) -- tl = id / Q,
+// i = id mod Q, Q = P/8 -- holds elements i + Q s (s = 0..7) of its transform tl on entry, and
+// the transform's outputs at the same positions on return.  The first pass (radix R0 = 8, 2
+// or 4 as log2 P mod 3 is 0, 1, 2; 8/R0 butterflies per thread, no twiddles) reads registers
+// only and the last radix-8 pass writes registers only, so a kernel that moves those positions
+// to and from HBM needs LDS only for the exchanges between passes (one up to 64 points, two up
+// to 512, three for 1024 and 2048).  wb is the owner's first transform slot; transform tl lives
+// at wb + at<LOGP>(tl, .).  rr holds the pass roots of this thread (load_roots<LOGP>(i, table)).
+// A block-owned transform synchronises the block: every thread of it must call.
 template <int LOGP>
 __device__ __forceinline__ void fft_regs(double2 (&u)[8], double2* wb, const regs_roots<LOGP>& rr) {
-    static_assert(LOGP >= 4 && LOGP <= 9, "register FFTs take 16..512 points");
+    static_assert(LOGP >= 4 && regs_of<LOGP>::ON, "register FFTs take 16..2048 points");
     constexpr int Q = 1 << (LOGP - 3);
     constexpr int LOGR0 = regs_roots<LOGP>::LOGR0;
     constexpr int R0 = 1 << LOGR0;
-    const int lane = threadIdx.x & 63, tl = lane >> (LOGP - 3), i = lane & (Q - 1);
+    const int id = regs_of<LOGP>::BLOCK ? (int)threadIdx.x : (int)(threadIdx.x & 63);
+    const int tl = id >> (LOGP - 3), i = id & (Q - 1);
     // first pass (p = 1): butterfly b = i + Q m takes elements b + (P/R0) r = u[m + (8/R0) r]
 #pragma unroll
     for (int m = 0; m < 8 / R0; ++m) {
@@ -282,9 +300,27 @@ __device__ __forceinline__ void fft_regs(double2 (&u)[8], double2* wb, const reg
 #pragma unroll
         for (int r = 0; r < R0; ++r) wb[at<LOGP>(tl, (i + Q * m) * R0 + r)] = v[r];
     }
-    wave_sync();
+    regs_sync<regs_of<LOGP>::BLOCK>();
     fft_regs_passes<LOGP, LOGR0>(u, wb, tl, i, rr);
 }
+
+// Position of this thread in the register FFTs of a 256-thread block: transform t (of the
+// block's 2048/P), its first transform t0 of the owning wave, and i within the transform.
+template <int LOGP>
+struct regs_pos {
+    int t0, t, i;
+    __device__ __forceinline__ regs_pos() {
+        constexpr int Q = 1 << (LOGP - 3);
+        if constexpr (regs_of<LOGP>::BLOCK) {
+            t0 = 0;
+            t = (int)threadIdx.x >> (LOGP - 3);
+        } else {
+            t0 = (int)(threadIdx.x >> 6) << (9 - LOGP);
+            t = t0 + ((int)(threadIdx.x & 63) >> (LOGP - 3));
+        }
+        i = (int)threadIdx.x & (Q - 1);
+    }
+};
 
 // rows_fwd: block (matrix, group of 2T rows); matrices 0..nl-1 are the left ones (into SL),
 // the rest the right ones (into SR).  Matrix i starts at element moff[i]
@@ -307,12 +343,12 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
     const uint32_t r0 = g << (logT + 1);
     const Tin* Xm = (left ? XL : XR) + (moff ? moff[matg] : (size_t)mat * H * W);
     double2* S = left ? SL : SR;
-    if constexpr (LOGPC <= 9) {
-        // T = 2048/Pc: the wave's transforms (row pairs) go through fft_regs from HBM; the
-        // spectra are left in natural order in LDS for the block's split below.
+    if constexpr (regs_of<LOGPC>::ON) {
+        // T = 2048/Pc: the transforms (row pairs) go through fft_regs from HBM; the spectra
+        // are left in natural order in LDS for the block's split below.
         constexpr int Q = 1 << (LOGPC - 3);
-        const int lane = threadIdx.x & 63, tl = lane >> (LOGPC - 3), i = lane & (Q - 1);
-        const int t0 = (threadIdx.x >> 6) << (9 - LOGPC), t = t0 + tl;
+        const regs_pos<LOGPC> ps;
+        const int t0 = ps.t0, t = ps.t, i = ps.i;
         const uint32_t a = r0 + 2 * t, b = a + 1;
         const regs_roots<LOGPC> rr = load_roots<LOGPC>(i, tw);
         // Branch-free loads from clamped (in-bounds) addresses, all issued before the first
@@ -416,13 +452,13 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
     }
     const double2* Sm = S + (size_t)r * Kc * H;
     double2* Ll = Lhat + (size_t)l * Kc * Pr;
-    if constexpr (LOGPR <= 9) {
-        // C = 2048/Pr: the wave's columns go from HBM to HBM through fft_regs; the product with
+    if constexpr (regs_of<LOGPR>::ON) {
+        // C = 2048/Pr: the columns go from HBM to HBM through fft_regs; the product with
         // conj(Lhat) happens in registers between the two FFTs.
         constexpr int Q = 1 << (LOGPR - 3);
-        const int lane = threadIdx.x & 63, tl = lane >> (LOGPR - 3), i = lane & (Q - 1);
-        const int t0 = (threadIdx.x >> 6) << (9 - LOGPR);
-        const uint32_t k = k0 + t0 + tl;
+        const regs_pos<LOGPR> ps;
+        const int t0 = ps.t0, i = ps.i;
+        const uint32_t k = k0 + ps.t;
         const bool ok = k < Kc;
         double2* wb = buf + at<LOGPR>(t0, 0);
         const regs_roots<LOGPR> rr = load_roots<LOGPR>(i, tw);
@@ -544,11 +580,10 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
     // Groups wholly between the two halves of the result have nothing to write.
     if (out_row(rho0, H, Pr) < 0 && out_row(rho0 + rows2 - 1, H, Pr) < 0 && rho0 >= H) return;
     const double2* Zq = Z + (size_t)item * Kc * Pr;
-    const int lane = threadIdx.x & 63;
     regs_roots<LOGPC> rr{};
     slab_roots<LOGPC> sr{};
-    if constexpr (LOGPC <= 9)
-        rr = load_roots<LOGPC>(lane & ((1 << (LOGPC - 3)) - 1), tw);
+    if constexpr (regs_of<LOGPC>::ON)
+        rr = load_roots<LOGPC>((int)threadIdx.x & ((1 << (LOGPC - 3)) - 1), tw);
     else
         sr = load_slab_roots<LOGPC>(tw);
     // One thread per (column kk, transform t) loads the two rows' values A[kk], B[kk] (32
@@ -583,15 +618,17 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
     }
     __syncthreads();
     Tout* Oq = O + (ooff ? ooff[q0 + item] : (size_t)(q0 + item) * (2 * H - 1) * SX);
-    if constexpr (LOGPC <= 9) {
-        // T = 2048/Pc: the wave's transforms (row pairs) go through fft_regs and are written
-        // from registers: lane (tl, i) holds circular columns c = i + Q r of rows rho0 + 2t, +1.
+    if constexpr (regs_of<LOGPC>::ON) {
+        // T = 2048/Pc: the transforms (row pairs) go through fft_regs and are written from
+        // registers: thread (t, i) holds circular columns c = i + Q r of rows rho0 + 2t, +1.
         constexpr int Q = 1 << (LOGPC - 3);
-        const int tl = lane >> (LOGPC - 3), i = lane & (Q - 1);
-        const int t0 = (threadIdx.x >> 6) << (9 - LOGPC), t = t0 + tl;
+        const regs_pos<LOGPC> ps;
+        const int t0 = ps.t0, t = ps.t, i = ps.i;
         double2 u[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) u[r] = buf[at<LOGPC>(t, i + Q * r)];
+        // a block-owned transform's first pass overwrites slots other waves still read
+        if constexpr (regs_of<LOGPC>::BLOCK) __syncthreads();
         fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), rr);
         const int oa = out_row(rho0 + 2 * t, H, Pr), ob = out_row(rho0 + 2 * t + 1, H, Pr);
 #pragma unroll
@@ -611,7 +648,7 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
         if (oy < 0) continue;
         Tout* orow = Oq + (size_t)oy * SX;
         const int t = rr >> 1;
-        for (int ox = lane; ox < (int)SX; ox += 64) {
+        for (int ox = threadIdx.x & 63; ox < (int)SX; ox += 64) {
             const int c = (ox - (int)(W - 1)) & (Pc - 1);
             const double2 f = buf[at<LOGPC>(t, c)];
             orow[ox] = (Tout)(((rr & 1) ? -f.y : f.x) * scale);
@@ -668,21 +705,19 @@ const double2* twiddles() {
 
 struct slab_shape {
     int logS;      // log2 of the slab (values per block)
-    bool wave;     // transforms owned by single waves
+    bool regs;     // register transforms (regs_of)
     int logT;      // log2 of the transforms per block
     uint32_t lds;  // dynamic LDS bytes
 };
 
 // The slab of slab_of<LOGP> and the transforms per block: as many as `want` asks for, within
-// the slab.  Transforms of up to 512 points are wave-owned (T >= 4 then fits the 2048-value
-// slab).
+// the slab.  Register transforms (up to 2048 points) always fill it: 8 values per thread, so a
+// 256-thread block holds 2048/P of them.
 slab_shape shape_for(int logP, uint32_t want) {
     slab_shape v{};
     v.logS = std::max(11, logP);  // as slab_of<LOGP>
-    v.wave = logP <= 9;
-    // wave-owned lengths use the full slab: the register FFTs give each lane 8 values of one
-    // transform, 512/P transforms per wave
-    int lt = v.wave ? v.logS - logP : 0;
+    v.regs = logP <= 11;          // as regs_of<LOGP>::ON
+    int lt = v.regs ? v.logS - logP : 0;
     while (lt + logP < v.logS && (1u << lt) < want) ++lt;
     v.logT = lt;
     v.lds = (uint32_t)(((1u << lt) * ((1u << logP) + (1u << logP) / 8 + 1)) * sizeof(double2));
