@@ -196,6 +196,12 @@ def main():
             log(f"c4 {nm}: {ms:.4f} ms err {res[nm]['max_norm_err']:.2e}")
         plan.close()
         fplan.close()
+        # AUTO batch: the schedule the batch autotuner measured fastest for these pairs
+        best, table = x.batch_autotune(pairs, L, R, O)
+        O.zero_()
+        ms = timed(lambda: x.batch(pairs, L, R, O, precision="auto"), args.steps, args.warmup)
+        res["batch_auto"] = {"ms": ms, "pairs_per_s": 512 / (ms * 1e-3), "max_norm_err": err_of(), "launches": 1}
+        log(f"c4 batch_auto: {ms:.4f} ms err {res['batch_auto']['max_norm_err']:.2e} (schedule {best})")
         for m in ["fp32", "bf16x3", "fft64k", "auto"]:
             prec, tile = METH[m]
 
@@ -215,6 +221,9 @@ def main():
         line = {"config": "c4", "workload": "uneven n_to_mn, 128 groups of 1+4 square matrices, sizes "
                 "round(16*32^U) seed 2024, f32 U[-1,1)", "pairs": 512, "sizes_min_max": [min(sizes), max(sizes)],
                 "methods": res, "auto_picks_per_group": picks,
+                "batch_autotune": {"best (precision, tile, branches)": best,
+                                   "ms per schedule": [[list(sc), t] for sc, t in table]},
+                "batch_auto_vs_fp32_direct": res["per_class_fp32"]["ms"] / res["batch_auto"]["ms"],
                 "auto_vs_fp32_direct": fp32 / res["per_class_auto"]["ms"],
                 "batch_graph_vs_per_class_bf16x3": res["per_class_bf16x3"]["ms"] / res["batch_bf16x3_graph"]["ms"],
                 "batch_graph_vs_fp32_direct": fp32 / res["batch_bf16x3_graph"]["ms"],

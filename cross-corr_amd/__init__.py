@@ -84,6 +84,9 @@ SIGNATURES = {
                                        ctypes.POINTER(ctypes.c_void_p)]),
     "xc_batch_plan_exec_f32": (_st, [_vp, _vp, _vp, _vp, _i32, _vp]),
     "xc_batch_plan_destroy": (_st, [_vp]),
+    "xc_batch_autotune_f32": (_st, [ctypes.POINTER(Pair), _u32, _vp, _vp, _vp, ctypes.POINTER(Options),
+                                    ctypes.POINTER(ctypes.c_float), _u32, ctypes.POINTER(_u32),
+                                    ctypes.POINTER(Options), _vp]),
     "xc_scatter_f32": (_st, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _vp]),
     "xc_scatter_f64": (_st, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _u32, _vp]),
     "xc_hadamard_f32": (_st, [_vp, _vp, _u64, _u32, _vp]),
@@ -234,14 +237,28 @@ def batch(pairs, left_flat, right_flat, out_flat, precision="auto", stream=None)
     return out_flat
 
 
+def batch_autotune(pairs, left_flat, right_flat, out_flat, stream=None):
+    """Times every candidate schedule of this pair list (out_flat is overwritten) and makes
+    the fastest the one AUTO batches of the same pairs use.  Returns (best, [(candidate, ms)])
+    with each schedule as (precision, tile, split)."""
+    arr = (Pair * len(pairs))(*[Pair(*p) for p in pairs])
+    cap = 16
+    cands, ms = (Options * cap)(), (ctypes.c_float * cap)()
+    count, best = ctypes.c_uint32(), Options()
+    check(lib().xc_batch_autotune_f32(arr, len(pairs), _ptr(left_flat), _ptr(right_flat), _ptr(out_flat), cands, ms,
+                                      cap, ctypes.byref(count), ctypes.byref(best), _stream(stream)))
+    sched = lambda o: (o.precision, o.tile, o.split)
+    return sched(best), [(sched(cands[i]), float(ms[i])) for i in range(min(count.value, cap))]
+
+
 class BatchPlan:
     """Reusable uneven-batch plan, replayed as a HIP graph."""
 
-    def __init__(self, pairs, precision="bf16x3", border=0, tile=0):
+    def __init__(self, pairs, precision="bf16x3", border=0, tile=0, branches=0):
         """precision bf16x3 / bf16 / fp16: one persistent MFMA kernel; fft64 with tile=1: the
-        own FFT kernels per size class, on parallel graph branches."""
+        own FFT kernels per size class, on `branches` parallel graph branches (0 = 8)."""
         arr = (Pair * len(pairs))(*[Pair(*p) for p in pairs])
-        opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, tile, 0, border)
+        opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, tile, branches, border)
         self._h = ctypes.c_void_p()
         check(lib().xc_batch_plan_create_f32(arr, len(pairs), ctypes.byref(opt), ctypes.byref(self._h)))
 

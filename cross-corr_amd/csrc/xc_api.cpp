@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "xc_internal.hpp"
 #include "xc_methods.hpp"
@@ -148,6 +149,23 @@ xc_status xc_batch_plan_exec_f32(xc_batch_plan* plan, const float* left, const f
 
 xc_status xc_batch_plan_destroy(xc_batch_plan* plan) {
     return guarded([&] { xc::batch_plan_destroy(plan); });
+}
+
+xc_status xc_batch_autotune_f32(const xc_pair* pairs, uint32_t num_pairs, const float* left, const float* right,
+                                float* out, xc_options* candidates, float* ms, uint32_t capacity, uint32_t* count,
+                                xc_options* best, xc_stream_t stream) {
+    return guarded([&] {
+        if (!pairs || !left || !right || !out) xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        std::vector<xc_options> c;
+        std::vector<float> t;
+        const uint32_t b = xc::batch_autotune_f32(pairs, num_pairs, left, right, out, (hipStream_t)stream, c, t);
+        for (uint32_t i = 0; i < capacity && i < c.size(); ++i) {
+            if (candidates) candidates[i] = c[i];
+            if (ms) ms[i] = t[i];
+        }
+        if (count) *count = (uint32_t)c.size();
+        if (best) *best = c[b];
+    });
 }
 
 xc_status xc_scatter_f32(const float* src, float* dst, uint32_t src_rows, uint32_t src_cols, uint32_t num_matrices,

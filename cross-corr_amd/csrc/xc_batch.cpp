@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <list>
 #include <map>
@@ -99,7 +101,8 @@ void* plan_alloc(xc_batch_plan* plan, size_t bytes) {
 
 // Own-FFT plan: pairs grouped by size class; per class the distinct left and right matrices
 // (by offset) are transformed once and every pair of the class is one result of the job.
-xc_batch_plan* fft_plan_create(const xc_pair* pairs, uint32_t num_pairs) {
+// Classes are dealt to `branches` (1..8) parallel graph branches.
+xc_batch_plan* fft_plan_create(const xc_pair* pairs, uint32_t num_pairs, uint32_t branches) {
     auto plan = std::make_unique<xc_batch_plan>();
     plan->fft = true;
     plan->kind = XC_PREC_FFT64;
@@ -169,8 +172,8 @@ xc_batch_plan* fft_plan_create(const xc_pair* pairs, uint32_t num_pairs) {
         plan->jobs[c].ooff = d64 + spans[c].ooff;
         plan->jobs[c].ptab = d32 + spans[c].ptab;
     }
-    // Classes, largest first, to the least loaded of up to 8 branches.
-    const uint32_t nb = (uint32_t)std::min<size_t>(8, plan->jobs.size());
+    // Classes, largest first, to the least loaded branch.
+    const uint32_t nb = (uint32_t)std::max<size_t>(1, std::min<size_t>(branches, plan->jobs.size()));
     std::vector<size_t> order(plan->jobs.size());
     for (size_t c = 0; c < order.size(); ++c) order[c] = c;
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
@@ -236,7 +239,8 @@ xc_batch_plan* batch_plan_create(const xc_pair* pairs, uint32_t num_pairs, const
     check_pairs(pairs, num_pairs);
     if (opt.precision == XC_PREC_FFT64) {
         if (opt.tile != 1) fail(XC_ERR_INVALID_ARGUMENT, "FFT batch plans run the own FFT kernels (tile 1)");
-        return fft_plan_create(pairs, num_pairs);
+        if (opt.split > 8) fail(XC_ERR_INVALID_ARGUMENT, "FFT batch plans take 1..8 branches (split; 0 = 8)");
+        return fft_plan_create(pairs, num_pairs, opt.split ? opt.split : 8);
     }
     const int kind = opt.precision == XC_PREC_AUTO ? XC_PREC_BF16X3 : (int)opt.precision;
     if (kind != XC_PREC_BF16X3 && kind != XC_PREC_BF16 && kind != XC_PREC_FP16)
@@ -400,7 +404,92 @@ void batch_plan_exec(xc_batch_plan* plan, const float* L, const float* R, float*
 
 void batch_plan_destroy(xc_batch_plan* plan) { delete plan; }
 
+namespace {
+
+std::string pairs_key(const xc_pair* pairs, uint32_t num_pairs) {
+    std::string key(reinterpret_cast<const char*>(pairs), sizeof(xc_pair) * num_pairs);
+    int dev = 0;
+    XC_HIP(hipGetDevice(&dev));
+    key.append(reinterpret_cast<const char*>(&dev), sizeof(dev));
+    return key;
+}
+
+std::mutex g_tuned_mu;
+std::map<std::string, xc_options> g_tuned_batches;  // pair list + device -> fastest schedule
+
+}  // namespace
+
+// Schedules the batch autotuner times: the own-FFT plan over 1, 2, 4 and 8 parallel branches
+// (when every size fits it, rows and cols <= 2048) and the persistent bf16x3 MFMA plan.  The
+// MFMA plan is left out when the FFT plan fits and the definition's work exceeds 1e13 MACs
+// (one execution would take about a second; the own FFT wins such batches by 10-100x).
+std::vector<xc_options> batch_candidates(const xc_pair* pairs, uint32_t num_pairs) {
+    bool fits = true;
+    double macs = 0.0;
+    for (uint32_t i = 0; i < num_pairs; ++i) {
+        fits = fits && pairs[i].rows <= 2048 && pairs[i].cols <= 2048;
+        macs += (double)pairs[i].rows * pairs[i].rows * pairs[i].cols * pairs[i].cols;
+    }
+    std::vector<xc_options> c;
+    if (fits)
+        for (uint32_t b : {1u, 2u, 4u, 8u}) {
+            xc_options o{};
+            o.precision = XC_PREC_FFT64;
+            o.tile = 1;
+            o.split = b;
+            c.push_back(o);
+        }
+    if (!fits || macs <= 1e13) {
+        xc_options m{};
+        m.precision = XC_PREC_BF16X3;
+        c.push_back(m);
+    }
+    return c;
+}
+
+// Times every candidate schedule on the caller's buffers (plan build and graph capture
+// untimed, then `reps` graph replays) and remembers the fastest for this pair list; `out` is
+// overwritten.  Returns the index of the fastest candidate.
+uint32_t batch_autotune_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, const float* R, float* O,
+                            hipStream_t s, std::vector<xc_options>& cands, std::vector<float>& ms) {
+    check_pairs(pairs, num_pairs);
+    if (num_pairs == 0) fail(XC_ERR_INVALID_ARGUMENT, "empty batch");
+    cands = batch_candidates(pairs, num_pairs);
+    ms.assign(cands.size(), INFINITY);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    XC_HIP(hipEventCreate(&e0));
+    XC_HIP(hipEventCreate(&e1));
+    struct ev_guard {
+        hipEvent_t a, b;
+        ~ev_guard() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{e0, e1};
+    const uint32_t reps = 3;
+    for (size_t c = 0; c < cands.size(); ++c) {
+        std::unique_ptr<xc_batch_plan, void (*)(xc_batch_plan*)> plan(batch_plan_create(pairs, num_pairs, cands[c]),
+                                                                      batch_plan_destroy);
+        batch_plan_exec(plan.get(), L, R, O, s, true);  // capture + warm-up
+        XC_HIP(hipEventRecord(e0, s));
+        for (uint32_t r = 0; r < reps; ++r) batch_plan_exec(plan.get(), L, R, O, s, true);
+        XC_HIP(hipEventRecord(e1, s));
+        XC_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        XC_HIP(hipEventElapsedTime(&t, e0, e1));
+        ms[c] = t / reps;
+    }
+    const uint32_t best = (uint32_t)(std::min_element(ms.begin(), ms.end()) - ms.begin());
+    std::lock_guard<std::mutex> lock(g_tuned_mu);
+    if (g_tuned_batches.size() > 64) g_tuned_batches.clear();
+    g_tuned_batches[pairs_key(pairs, num_pairs)] = cands[best];
+    return best;
+}
+
 // One-shot batch: plans are cached by their pair list and options (a few per process).
+// AUTO takes the schedule the batch autotuner measured fastest for this pair list (running
+// it on the call's buffers the first time), or with XC_AUTO=model the own-FFT plan when
+// every size fits it, else bf16x3.
 void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, const float* R, float* O,
                    const xc_options& opt, hipStream_t s) {
     if (opt.precision == XC_PREC_FP32) {
@@ -413,20 +502,32 @@ void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, con
     }
     xc_options o = opt;
     if (o.precision == XC_PREC_AUTO) {
-        // The own FFT method is the faster one at every size class it takes (rows, cols <= 2048).
-        bool fits = true;
-        for (uint32_t i = 0; i < num_pairs && fits; ++i) fits = pairs[i].rows <= 2048 && pairs[i].cols <= 2048;
-        o = xc_options{};
-        o.precision = fits ? XC_PREC_FFT64 : XC_PREC_BF16X3;
-        o.tile = fits ? 1 : 0;
+        bool have = false;
+        {
+            std::lock_guard<std::mutex> lock(g_tuned_mu);
+            auto it = g_tuned_batches.find(pairs_key(pairs, num_pairs));
+            if (it != g_tuned_batches.end()) {
+                o = it->second;
+                have = true;
+            }
+        }
+        if (!have) {
+            const char* mode = std::getenv("XC_AUTO");
+            if (mode && std::string(mode) == "model") {
+                // the first candidate is the own-FFT plan when every size fits it, else bf16x3
+                o = batch_candidates(pairs, num_pairs).front();
+                if (o.precision == XC_PREC_FFT64) o.split = 8;
+            } else {
+                std::vector<xc_options> cands;
+                std::vector<float> ms;
+                o = cands[batch_autotune_f32(pairs, num_pairs, L, R, O, s, cands, ms)];
+            }
+        }
     }
     static std::mutex mu;
     static std::list<std::pair<std::string, std::unique_ptr<xc_batch_plan>>> cache;
     std::string key(reinterpret_cast<const char*>(&o), sizeof(o));
-    key.append(reinterpret_cast<const char*>(pairs), sizeof(xc_pair) * num_pairs);
-    int dev = 0;
-    XC_HIP(hipGetDevice(&dev));
-    key.append(reinterpret_cast<const char*>(&dev), sizeof(dev));
+    key.append(pairs_key(pairs, num_pairs));
     std::lock_guard<std::mutex> lock(mu);
     xc_batch_plan* plan = nullptr;
     for (auto it = cache.begin(); it != cache.end(); ++it)

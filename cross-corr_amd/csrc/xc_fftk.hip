@@ -281,15 +281,16 @@ __device__ __forceinline__ void fft_regs_passes(double2 (&u)[8], double2* wb, in
 // to and from HBM needs LDS only for the exchanges between passes (one up to 64 points, two up
 // to 512, three for 1024 and 2048).  wb is the owner's first transform slot; transform tl lives
 // at wb + at<LOGP>(tl, .).  rr holds the pass roots of this thread (load_roots<LOGP>(i, table)).
-// A block-owned transform synchronises the block: every thread of it must call.
+// A block-owned transform synchronises the block: every thread of it must call.  (tl, i) are
+// passed in (regs_pos) so that a kernel calling twice can keep the compiler from holding the
+// first call's LDS addresses in registers for the second.
 template <int LOGP>
-__device__ __forceinline__ void fft_regs(double2 (&u)[8], double2* wb, const regs_roots<LOGP>& rr) {
+__device__ __forceinline__ void fft_regs(double2 (&u)[8], double2* wb, const regs_roots<LOGP>& rr, int tl,
+                                         int i) {
     static_assert(LOGP >= 4 && regs_of<LOGP>::ON, "register FFTs take 16..2048 points");
     constexpr int Q = 1 << (LOGP - 3);
     constexpr int LOGR0 = regs_roots<LOGP>::LOGR0;
     constexpr int R0 = 1 << LOGR0;
-    const int id = regs_of<LOGP>::BLOCK ? (int)threadIdx.x : (int)(threadIdx.x & 63);
-    const int tl = id >> (LOGP - 3), i = id & (Q - 1);
     // first pass (p = 1): butterfly b = i + Q m takes elements b + (P/R0) r = u[m + (8/R0) r]
 #pragma unroll
     for (int m = 0; m < 8 / R0; ++m) {
@@ -370,7 +371,7 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
             const bool in = i + Q * r < (int)W;
             u[r] = make_double2(oka && in ? (double)va[r] : 0.0, okb && in ? (double)vb[r] : 0.0);
         }
-        fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), rr);
+        fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), rr, t - t0, i);
 #pragma unroll
         for (int r = 0; r < 8; ++r) buf[at<LOGPC>(t, i + Q * r)] = u[r];
         __syncthreads();
@@ -477,7 +478,7 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
             lh[r] = make_double2(0.0, 0.0);
             if (mode == 1 && ok) lh[r] = lk[i + Q * r];
         }
-        fft_regs<LOGPR>(u, wb, rr);
+        fft_regs<LOGPR>(u, wb, rr, ps.t - t0, i);
         if (mode == 0) {
             if (ok) {
 #pragma unroll
@@ -487,7 +488,11 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
         }
 #pragma unroll
         for (int r = 0; r < 8; ++r) u[r] = cmul(cconj(lh[r]), u[r]);
-        fft_regs<LOGPR>(u, wb, rr);
+        // An opaque copy of the position: the second transform recomputes its LDS addresses
+        // instead of keeping the first one's live across the product (fewer VGPRs).
+        int tl2 = ps.t - t0, i2 = i;
+        asm volatile("" : "+v"(tl2), "+v"(i2));
+        fft_regs<LOGPR>(u, wb, rr, tl2, i2);
         if (ok) {
             double2* zk = Z + (size_t)item * Kc * Pr + (size_t)k * Pr;
 #pragma unroll
@@ -629,7 +634,7 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
         for (int r = 0; r < 8; ++r) u[r] = buf[at<LOGPC>(t, i + Q * r)];
         // a block-owned transform's first pass overwrites slots other waves still read
         if constexpr (regs_of<LOGPC>::BLOCK) __syncthreads();
-        fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), rr);
+        fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), rr, t - t0, i);
         const int oa = out_row(rho0 + 2 * t, H, Pr), ob = out_row(rho0 + 2 * t + 1, H, Pr);
 #pragma unroll
         for (int r = 0; r < 8; ++r) {

@@ -47,6 +47,9 @@ uint32_t batch_blocks_per_cu();
 void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, const float* R, float* O,
                    const xc_options& opt, hipStream_t s);
 xc_batch_plan* batch_plan_create(const xc_pair* pairs, uint32_t num_pairs, const xc_options& opt);
+std::vector<xc_options> batch_candidates(const xc_pair* pairs, uint32_t num_pairs);
+uint32_t batch_autotune_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, const float* R, float* O,
+                            hipStream_t s, std::vector<xc_options>& cands, std::vector<float>& ms);
 void batch_plan_destroy(xc_batch_plan* plan);  // runs the plan's destructor
 void batch_plan_exec(xc_batch_plan* plan, const float* L, const float* R, float* O, hipStream_t s, bool use_graph);
 

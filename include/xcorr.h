@@ -109,21 +109,43 @@ typedef struct xc_pair {
     uint32_t rows, cols;
 } xc_pair;
 
+/* opt->precision: XC_PREC_FP32 (one direct launch per pair), an MFMA method or
+ * XC_PREC_FFT64 tile 1 (a plan as below), or XC_PREC_AUTO: the schedule that
+ * xc_batch_autotune_f32 measured fastest for this pair list, measured on this call's buffers
+ * the first time (XC_AUTO=model: the own-FFT plan when every size fits it, else bf16x3). */
 xc_status xc_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* left, const float* right,
                        float* out, const xc_options* opt, xc_stream_t stream);
 
-/* Reusable batch plan: the work list (MFMA block tiles of every pair, largest first, plus
- * border-frame chunks for bf16x3) is built and uploaded once; each execution is one launch
- * of the persistent kernel, captured into a HIP graph on first use and replayed (kernel
- * arguments are updated in the instantiated graph when the matrix pointers change).
+/* Reusable batch plan, captured into a HIP graph on first use and replayed.
+ *  - bf16x3 / bf16 / fp16: the work list (MFMA block tiles of every pair, largest first, plus
+ *    border-frame chunks for bf16x3) is built and uploaded once; each execution is one launch
+ *    of the persistent kernel (kernel arguments are updated in the instantiated graph when
+ *    the matrix pointers change).
+ *  - XC_PREC_FFT64 tile 1: the own FFT kernels per size class (same rows x cols; a matrix
+ *    shared by several pairs is transformed once), classes dealt largest first to opt->split
+ *    parallel graph branches (1..8, 0 = 8), each with its own workspaces; re-captured when
+ *    the matrix pointers change.  rows, cols <= 2048.
  * xc_batch_f32 keeps a few such plans cached by pair list.  A plan must not be executed
- * concurrently with itself (its work counter is shared); destroy synchronises the device. */
+ * concurrently with itself (its work counter and workspaces are shared); destroy
+ * synchronises the device. */
 typedef struct xc_batch_plan xc_batch_plan;
 xc_status xc_batch_plan_create_f32(const xc_pair* pairs, uint32_t num_pairs, const xc_options* opt,
                                    xc_batch_plan** plan);
 xc_status xc_batch_plan_exec_f32(xc_batch_plan* plan, const float* left, const float* right, float* out,
                                  int use_graph, xc_stream_t stream);
 xc_status xc_batch_plan_destroy(xc_batch_plan* plan);
+
+/* Batch autotuner: builds and times each candidate schedule of this pair list on the
+ * caller's buffers -- the own-FFT plan over 1, 2, 4 and 8 parallel branches (when rows and
+ * cols <= 2048) and the persistent bf16x3 MFMA plan (left out when the FFT plan fits and the
+ * definition's work exceeds 1e13 MACs) -- as graph replays after an untimed
+ * capture, remembers the fastest for XC_PREC_AUTO batches of the same pair list and returns
+ * it in *best.  candidates and ms (either may be null) receive the first `capacity`
+ * candidates and their mean milliseconds per execution; *count (may be null) the number of
+ * candidates.  `out` is overwritten; the stream is synchronised. */
+xc_status xc_batch_autotune_f32(const xc_pair* pairs, uint32_t num_pairs, const float* left, const float* right,
+                                float* out, xc_options* candidates, float* ms, uint32_t capacity, uint32_t* count,
+                                xc_options* best, xc_stream_t stream);
 
 /* ---- FFT-method building blocks (src/kernels.cuh:7-63) ------------------------------ */
 /* run_scatter (src/kernels.cuh:53-63, src/scatter.cu:15-102): copy src matrices into

@@ -180,3 +180,49 @@ def test_batch_fft_plan_rejects_oversize():
     with pytest.raises(x.XcError) as e:
         x.BatchPlan([(0, 0, 0, 2049, 4)], "fft64", tile=1)
     assert e.value.status == 4
+
+
+def test_batch_autotune_times_every_schedule_and_auto_replays_the_fastest():
+    """The batch autotuner times the own-FFT plan over 1, 2, 4 and 8 branches and the bf16x3
+    plan on the call's buffers; an AUTO batch of the same pairs then runs the fastest."""
+    x = xcorr()
+    lf, rf, pairs, olen, sizes = uneven_batch(24, 9, lo=16, hi=300)
+    L, R = dev(lf), dev(rf)
+    out = torch.full((olen,), float("nan"), device="cuda")
+    best, table = x.batch_autotune(pairs, L, R, out)
+    assert [s for s, _ in table] == [(5, 1, 1), (5, 1, 2), (5, 1, 4), (5, 1, 8), (2, 0, 0)]
+    assert all(np.isfinite(ms) and ms > 0 for _, ms in table)
+    assert best == min(table, key=lambda t: t[1])[0]
+    fft_best = min(ms for s, ms in table if s[0] == 5)
+    assert fft_best < dict(table)[(2, 0, 0)], table  # the own FFT beats the definition here
+    out2 = torch.full((olen,), float("nan"), device="cuda")
+    x.batch(pairs, L, R, out2, precision="auto")
+    torch.cuda.synchronize()
+    ll = x.last_launch()
+    assert ll["kernel"] == "fft batch graph" and ll["split"] == best[2], (ll, best)
+    got = out2.cpu().numpy()
+    assert not np.isnan(got).any()
+    order = np.argsort(sizes)
+    batch_oracle_check(lf, rf, pairs, got, 1e-6, check_pairs=[4 * g + j for g in order[:6] for j in (0, 3)])
+
+
+def test_batch_fft_plan_branch_counts_agree():
+    """The branch count only changes the schedule: every count gives the same bits."""
+    x = xcorr()
+    lf, rf, pairs, olen, _ = uneven_batch(10, 13, lo=16, hi=200)
+    L, R = dev(lf), dev(rf)
+    ref = None
+    for b in (1, 3, 8):
+        plan = x.BatchPlan(pairs, "fft64", tile=1, branches=b)
+        out = torch.full((olen,), float("nan"), device="cuda")
+        plan.run(L, R, out, use_graph=True)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        plan.close()
+        if ref is None:
+            batch_oracle_check(lf, rf, pairs, got, 1e-6, check_pairs=list(range(0, len(pairs), 5)))
+            ref = got
+        else:
+            np.testing.assert_array_equal(got, ref)
+    with pytest.raises(x.XcError):
+        x.BatchPlan(pairs, "fft64", tile=1, branches=9)
