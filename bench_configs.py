@@ -197,9 +197,10 @@ def main():
         plan.close()
         fplan.close()
         # AUTO batch: the schedule the batch autotuner measured fastest for these pairs
-        best, table = x.batch_autotune(pairs, L, R, O)
+        parr = x.pair_array(pairs)
+        best, table = x.batch_autotune(parr, L, R, O)
         O.zero_()
-        ms = timed(lambda: x.batch(pairs, L, R, O, precision="auto"), args.steps, args.warmup)
+        ms = timed(lambda: x.batch(parr, L, R, O, precision="auto"), args.steps, args.warmup)
         res["batch_auto"] = {"ms": ms, "pairs_per_s": 512 / (ms * 1e-3), "max_norm_err": err_of(), "launches": 1}
         log(f"c4 batch_auto: {ms:.4f} ms err {res['batch_auto']['max_norm_err']:.2e} (schedule {best})")
         for m in ["fp32", "bf16x3", "fft64k", "auto"]:

@@ -228,9 +228,18 @@ def fft_cross_corr(family, left, right, out=None, normalize=True, fp64_transform
     return out
 
 
+def pair_array(pairs):
+    """ctypes array of xc_pair from (left_off, right_off, out_off, rows, cols) tuples; pass it
+    instead of the list to calls made many times (building it costs ~1 us per pair)."""
+    if isinstance(pairs, ctypes.Array):
+        return pairs
+    return (Pair * len(pairs))(*[Pair(*p) for p in pairs])
+
+
 def batch(pairs, left_flat, right_flat, out_flat, precision="auto", stream=None):
-    """Uneven batch.  pairs: list of (left_off, right_off, out_off, rows, cols) in elements."""
-    arr = (Pair * len(pairs))(*[Pair(*p) for p in pairs])
+    """Uneven batch.  pairs: list of (left_off, right_off, out_off, rows, cols) in elements,
+    or a pair_array."""
+    arr = pair_array(pairs)
     opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, 0, 0, 0)
     check(lib().xc_batch_f32(arr, len(pairs), _ptr(left_flat), _ptr(right_flat), _ptr(out_flat),
                              ctypes.byref(opt), _stream(stream)))
@@ -241,7 +250,7 @@ def batch_autotune(pairs, left_flat, right_flat, out_flat, stream=None):
     """Times every candidate schedule of this pair list (out_flat is overwritten) and makes
     the fastest the one AUTO batches of the same pairs use.  Returns (best, [(candidate, ms)])
     with each schedule as (precision, tile, split)."""
-    arr = (Pair * len(pairs))(*[Pair(*p) for p in pairs])
+    arr = pair_array(pairs)
     cap = 16
     cands, ms = (Options * cap)(), (ctypes.c_float * cap)()
     count, best = ctypes.c_uint32(), Options()
@@ -257,7 +266,7 @@ class BatchPlan:
     def __init__(self, pairs, precision="bf16x3", border=0, tile=0, branches=0):
         """precision bf16x3 / bf16 / fp16: one persistent MFMA kernel; fft64 with tile=1: the
         own FFT kernels per size class, on `branches` parallel graph branches (0 = 8)."""
-        arr = (Pair * len(pairs))(*[Pair(*p) for p in pairs])
+        arr = pair_array(pairs)
         opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, tile, branches, border)
         self._h = ctypes.c_void_p()
         check(lib().xc_batch_plan_create_f32(arr, len(pairs), ctypes.byref(opt), ctypes.byref(self._h)))
