@@ -211,7 +211,7 @@ __device__ __forceinline__ void fft_slab(double2* buf, int T, const slab_roots<L
 
 // Transforms of up to 2048 points run in registers: P/8 threads hold one transform, 8 values
 // each.  Up to 512 points a transform lives in one wave (BLOCK false: the wave holds 512/P of
-// them); 1024 and 2048 points span two and four waves, so a 256-thread block holds 2048/P.
+// them); 1024 and 2048 points span two and four waves of the block.
 template <int LOGP>
 struct regs_of {
     static constexpr bool ON = LOGP <= 11;
@@ -305,8 +305,8 @@ __device__ __forceinline__ void fft_regs(double2 (&u)[8], double2* wb, const reg
     fft_regs_passes<LOGP, LOGR0>(u, wb, tl, i, rr);
 }
 
-// Position of this thread in the register FFTs of a 256-thread block: transform t (of the
-// block's 2048/P), its first transform t0 of the owning wave, and i within the transform.
+// Position of this thread in the register FFTs of its block: transform t, the first transform
+// t0 of the owning wave (0 for block-owned transforms), and i within the transform.
 template <int LOGP>
 struct regs_pos {
     int t0, t, i;
@@ -323,11 +323,24 @@ struct regs_pos {
     }
 };
 
+// Threads per block of the row kernels (INV: rows_inv): 256, or more for register transforms
+// of 1024 and 2048 points so that a block holds four transforms (8 rows: a column's piece of
+// S or Z is a whole 128-byte line) -- except rows_inv at 2048 points, where two transforms per
+// block (two blocks per CU instead of one) measured faster.  LOGS is the block's slab.
+constexpr int rows_threads(int logP, bool inv) {
+    return logP == 10 ? 512 : (logP == 11 ? (inv ? 512 : 1024) : 256);
+}
+template <int LOGP, bool INV>
+struct rows_block {
+    static constexpr int NT = rows_threads(LOGP, INV);
+    static constexpr int LOGS = regs_of<LOGP>::ON ? (NT == 256 ? 11 : (NT == 512 ? 12 : 13)) : slab_of<LOGP>::LOGS;
+};
+
 // rows_fwd: block (matrix, group of 2T rows); matrices 0..nl-1 are the left ones (into SL),
 // the rest the right ones (into SR).  Matrix i starts at element moff[i]
 // of its input when the offset table is given (uneven batches), else at i * H * W.
-template <typename Tin, int LOGPC>
-__global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ XL, const Tin* __restrict__ XR,
+template <typename Tin, int LOGPC, int NT = rows_block<LOGPC, false>::NT>
+__global__ __launch_bounds__(NT) void rows_fwd_kernel(const Tin* __restrict__ XL, const Tin* __restrict__ XR,
                                                        const uint64_t* __restrict__ moff, uint32_t nl, uint32_t H,
                                                        uint32_t W, int logT, uint32_t groups,
                                                        double2* __restrict__ SL, double2* __restrict__ SR,
@@ -345,7 +358,7 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
     const Tin* Xm = (left ? XL : XR) + (moff ? moff[matg] : (size_t)mat * H * W);
     double2* S = left ? SL : SR;
     if constexpr (regs_of<LOGPC>::ON) {
-        // T = 2048/Pc: the transforms (row pairs) go through fft_regs from HBM; the spectra
+        // T = 8 NT / Pc: the transforms (row pairs) go through fft_regs from HBM; the spectra
         // are left in natural order in LDS for the block's split below.
         constexpr int Q = 1 << (LOGPC - 3);
         const regs_pos<LOGPC> ps;
@@ -406,7 +419,7 @@ __global__ __launch_bounds__(kFT) void rows_fwd_kernel(const Tin* __restrict__ X
     const uint32_t Kc = (uint32_t)(Pc / 2 + 1);
     const int rows2 = 2 << logT;
     double2* Sm = S + (size_t)mat * Kc * H;
-    for (int idx = threadIdx.x; idx < (int)Kc * rows2; idx += kFT) {
+    for (int idx = threadIdx.x; idx < (int)Kc * rows2; idx += NT) {
         const int k = idx >> (logT + 1), rr = idx & (rows2 - 1);
         const uint32_t r = r0 + rr;
         if (r >= H) continue;
@@ -564,8 +577,8 @@ __device__ __forceinline__ int out_row(uint32_t rho, uint32_t H, uint32_t Pr) {
 // goes to output row out_row(rho); output column ox = dx + W - 1 takes circular column
 // dx mod Pc.  Pair q's result starts at ooff[q] when the table is given, else at q times the
 // result size.
-template <typename Tout, int LOGPC>
-__global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict__ Z, Tout* __restrict__ O,
+template <typename Tout, int LOGPC, int NT = rows_block<LOGPC, true>::NT>
+__global__ __launch_bounds__(NT) void rows_inv_kernel(const double2* __restrict__ Z, Tout* __restrict__ O,
                                                        const uint64_t* __restrict__ ooff, uint32_t q0, uint32_t H,
                                                        uint32_t W, int logPr, int logT, uint32_t groups, double scale,
                                                        const double2* __restrict__ tw) {
@@ -597,13 +610,13 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
     {
         // At most TMAX = 2^LOGS / Pc transforms, so (Kc << logT) <= (Pc/2 + 1) * TMAX items: all
         // loads first, then the writes.
-        constexpr int TMAX = (1 << slab_of<LOGPC>::LOGS) >> LOGPC;
-        constexpr int NQ = (((1 << (LOGPC - 1)) + 1) * TMAX + kFT - 1) / kFT;
+        constexpr int TMAX = (1 << rows_block<LOGPC, true>::LOGS) >> LOGPC;
+        constexpr int NQ = (((1 << (LOGPC - 1)) + 1) * TMAX + NT - 1) / NT;
         const int items = (int)Kc << logT;
         double2 A[NQ], B[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            const int idx = threadIdx.x + kFT * q;
+            const int idx = threadIdx.x + NT * q;
             const int kk = idx >> logT, t = idx & (T - 1);
             const uint32_t ra = rho0 + 2 * t;
             const double2* zc = Zq + (size_t)kk * Pr + ra;
@@ -614,7 +627,7 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
         }
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            const int idx = threadIdx.x + kFT * q;
+            const int idx = threadIdx.x + NT * q;
             if (idx >= items) continue;
             const int kk = idx >> logT, t = idx & (T - 1);
             buf[at<LOGPC>(t, kk)] = cadd(cconj(A[q]), mul_mi(cconj(B[q])));
@@ -624,7 +637,7 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
     __syncthreads();
     Tout* Oq = O + (ooff ? ooff[q0 + item] : (size_t)(q0 + item) * (2 * H - 1) * SX);
     if constexpr (regs_of<LOGPC>::ON) {
-        // T = 2048/Pc: the transforms (row pairs) go through fft_regs and are written from
+        // T = 8 NT / Pc: the transforms (row pairs) go through fft_regs and are written from
         // registers: thread (t, i) holds circular columns c = i + Q r of rows rho0 + 2t, +1.
         constexpr int Q = 1 << (LOGPC - 3);
         const regs_pos<LOGPC> ps;
@@ -648,7 +661,7 @@ __global__ __launch_bounds__(kFT) void rows_inv_kernel(const double2* __restrict
     } else {
     fft_slab<LOGPC>(buf, T, sr);
     // IFFT(Y) = conj(FFT(conj(Y))) / Pc: row A is the real part, row B minus the imaginary.
-    for (int rr = threadIdx.x >> 6; rr < rows2; rr += kFT / 64) {
+    for (int rr = threadIdx.x >> 6; rr < rows2; rr += NT / 64) {
         const int oy = out_row(rho0 + rr, H, Pr);
         if (oy < 0) continue;
         Tout* orow = Oq + (size_t)oy * SX;
@@ -717,11 +730,12 @@ struct slab_shape {
 
 // The slab of slab_of<LOGP> and the transforms per block: as many as `want` asks for, within
 // the slab.  Register transforms (up to 2048 points) always fill it: 8 values per thread, so a
-// 256-thread block holds 2048/P of them.
-slab_shape shape_for(int logP, uint32_t want) {
+// block of `threads` holds 8 threads / P of them.
+slab_shape shape_for(int logP, uint32_t want, int threads = 256) {
     slab_shape v{};
-    v.logS = std::max(11, logP);  // as slab_of<LOGP>
-    v.regs = logP <= 11;          // as regs_of<LOGP>::ON
+    v.regs = logP <= 11;  // as regs_of<LOGP>::ON
+    // as rows_block<LOGP>::LOGS for the row kernels (threads), slab_of<LOGP>::LOGS otherwise
+    v.logS = v.regs ? ilog2_ceil((uint32_t)threads) + 3 : std::max(11, logP);
     int lt = v.regs ? v.logS - logP : 0;
     while (lt + logP < v.logS && (1u << lt) < want) ++lt;
     v.logT = lt;
@@ -772,9 +786,9 @@ fftk_geom geom_of(uint32_t H, uint32_t W) {
     g.Kc = g.Pc / 2 + 1;
     // Slab shape per kernel (see shape_for): rows_fwd 2T input rows per block, cols C columns,
     // rows_inv 2T output rows.
-    g.v1 = shape_for(g.logPc, (H + 1) / 2);
+    g.v1 = shape_for(g.logPc, (H + 1) / 2, rows_threads(g.logPc, false));
     g.vc = shape_for(g.logPr, g.Kc);
-    g.v3 = shape_for(g.logPc, g.Pr / 2);
+    g.v3 = shape_for(g.logPc, g.Pr / 2, rows_threads(g.logPc, true));
     g.g1 = (H + (2u << g.v1.logT) - 1) >> (g.v1.logT + 1);
     g.cg = (g.Kc + (1u << g.vc.logT) - 1) >> g.vc.logT;
     g.g3 = (g.Pr + (2u << g.v3.logT) - 1) >> (g.v3.logT + 1);  // a block may be taller than Pr
@@ -801,7 +815,8 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
         with_logp(g.logPc, [&](auto LP) {
             auto k = rows_fwd_kernel<Tio, decltype(LP)::value>;
             allow_lds(k, g.v1.lds);
-            hipLaunchKernelGGL(k, dim3((j.nl + j.nr) * g.g1), dim3(kFT), g.v1.lds, s, L, R, j.moff, j.nl, H, W,
+            hipLaunchKernelGGL(k, dim3((j.nl + j.nr) * g.g1), dim3(rows_threads(g.logPc, false)), g.v1.lds, s, L, R, j.moff,
+                               j.nl, H, W,
                                g.v1.logT, g.g1, SL, SR, tw);
         });
     }
@@ -828,12 +843,13 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
         with_logp(g.logPc, [&](auto LP) {
             auto k = rows_inv_kernel<Tio, decltype(LP)::value>;
             allow_lds(k, g.v3.lds);
-            hipLaunchKernelGGL(k, dim3(nq * g.g3), dim3(kFT), g.v3.lds, s, (const double2*)Zw, O, j.ooff, q0, H, W,
+            hipLaunchKernelGGL(k, dim3(nq * g.g3), dim3(rows_threads(g.logPc, true)), g.v3.lds, s, (const double2*)Zw, O,
+                               j.ooff, q0, H, W,
                                g.logPr, g.v3.logT, g.g3, scale, tw);
         });
     }
     XC_HIP(hipGetLastError());
-    record_launch("fftk_rows_inv", dim3(g.g3), dim3(kFT), g.v3.lds, H, W, j.pairs, 1);
+    record_launch("fftk_rows_inv", dim3(g.g3), dim3(rows_threads(g.logPc, true)), g.v3.lds, H, W, j.pairs, 1);
 }
 
 template <typename Tio>
