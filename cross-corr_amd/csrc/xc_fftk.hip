@@ -255,7 +255,11 @@ __device__ __forceinline__ void fft_regs_passes(double2 (&u)[8], double2* wb, in
 #pragma unroll
     for (int r = 0; r < 8; ++r) u[r] = wb[at<LOGP>(tl, i + Q * r)];
     const int k = i & ((1 << LOGPP) - 1);
-    const double2 w1 = rr.w[(LOGPP - regs_roots<LOGP>::LOGR0) / 3];
+    // The root passes through an empty asm so that its powers are formed here, in this pass:
+    // otherwise the compiler hoists the power chains of every pass (and keeps them for a second
+    // transform with the same roots), which costs tens of VGPRs.
+    double2 w1 = rr.w[(LOGPP - regs_roots<LOGP>::LOGR0) / 3];
+    asm volatile("" : "+v"(w1.x), "+v"(w1.y));
     double2 w = w1;
 #pragma unroll
     for (int r = 1; r < 8; ++r) {
