@@ -13,6 +13,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "xc_methods.hpp"
@@ -99,9 +100,11 @@ void* plan_alloc(xc_batch_plan* plan, size_t bytes) {
     return p;
 }
 
-// Own-FFT plan: pairs grouped by size class; per class the distinct left and right matrices
-// (by offset) are transformed once and every pair of the class is one result of the job.
-// Classes are dealt to `branches` (1..8) parallel graph branches.
+// Own-FFT plan: pairs grouped into classes of equal padded lengths (Pr x Pc, the powers of two
+// >= 2 rows - 1 and 2 cols - 1), one job per class whose matrices keep their own sizes (the
+// job's per-matrix size table); per class the distinct left and right matrices (by offset and
+// size) are transformed once and every pair of the class is one result of the job.  Classes
+// are dealt to `branches` (1..8) parallel graph branches.
 xc_batch_plan* fft_plan_create(const xc_pair* pairs, uint32_t num_pairs, uint32_t branches) {
     auto plan = std::make_unique<xc_batch_plan>();
     plan->fft = true;
@@ -110,58 +113,80 @@ xc_batch_plan* fft_plan_create(const xc_pair* pairs, uint32_t num_pairs, uint32_
     plan->pairs.assign(pairs, pairs + num_pairs);
     XC_HIP(hipGetDevice(&plan->device));
     fftk_prepare();
+    using mkey = std::tuple<uint64_t, uint32_t, uint32_t>;  // offset, rows, cols
     struct cls {
         std::vector<uint64_t> loffs, roffs, ooffs;
-        std::vector<uint32_t> ptab;
-        std::map<uint64_t, uint32_t> lidx, ridx;
+        std::vector<uint32_t> ldims, rdims, ptab;
+        std::map<mkey, uint32_t> lidx, ridx;
+        uint32_t H = 0, W = 0;
+        double work = 0.0;
+    };
+    auto pad = [](uint32_t v) {
+        uint32_t p = 16;
+        while (p < 2 * v - 1) p <<= 1;
+        return p;
     };
     std::map<std::pair<uint32_t, uint32_t>, cls> classes;
     for (uint32_t i = 0; i < num_pairs; ++i) {
         const xc_pair& p = pairs[i];
         if (2 * p.rows - 1 > 4096 || 2 * p.cols - 1 > 4096)
             fail(XC_ERR_UNSUPPORTED, "pair " + std::to_string(i) + ": the FFT batch plan takes rows, cols <= 2048");
-        cls& c = classes[{p.rows, p.cols}];
-        auto li = c.lidx.emplace(p.left_off, (uint32_t)c.loffs.size());
-        if (li.second) c.loffs.push_back(p.left_off);
-        auto ri = c.ridx.emplace(p.right_off, (uint32_t)c.roffs.size());
-        if (ri.second) c.roffs.push_back(p.right_off);
+        cls& c = classes[{pad(p.rows), pad(p.cols)}];
+        auto li = c.lidx.emplace(mkey{p.left_off, p.rows, p.cols}, (uint32_t)c.loffs.size());
+        if (li.second) {
+            c.loffs.push_back(p.left_off);
+            c.ldims.insert(c.ldims.end(), {p.rows, p.cols});
+            c.work += (double)p.rows * p.cols;
+        }
+        auto ri = c.ridx.emplace(mkey{p.right_off, p.rows, p.cols}, (uint32_t)c.roffs.size());
+        if (ri.second) {
+            c.roffs.push_back(p.right_off);
+            c.rdims.insert(c.rdims.end(), {p.rows, p.cols});
+            c.work += (double)p.rows * p.cols;
+        }
         c.ptab.push_back(li.first->second);
         c.ptab.push_back(ri.first->second);
         c.ooffs.push_back(p.out_off);
+        c.H = std::max(c.H, p.rows);
+        c.W = std::max(c.W, p.cols);
+        c.work += 2.0 * p.rows * p.cols;
     }
     // One device buffer for every class's tables.
     size_t words64 = 0, words32 = 0;
     for (auto& kv : classes) {
         words64 += kv.second.loffs.size() + kv.second.roffs.size() + kv.second.ooffs.size();
-        words32 += kv.second.ptab.size();
+        words32 += kv.second.ptab.size() + kv.second.ldims.size() + kv.second.rdims.size();
     }
     std::vector<uint64_t> t64;
     std::vector<uint32_t> t32;
     t64.reserve(words64);
     t32.reserve(words32);
     struct span {
-        size_t moff, ooff, ptab;
+        size_t moff, ooff, ptab, dims;
     };
     std::vector<span> spans;
     std::vector<double> cost;
     for (auto& kv : classes) {
         cls& c = kv.second;
-        span sp{t64.size(), 0, t32.size()};
+        span sp{t64.size(), 0, t32.size(), 0};
         t64.insert(t64.end(), c.loffs.begin(), c.loffs.end());
         t64.insert(t64.end(), c.roffs.begin(), c.roffs.end());
         sp.ooff = t64.size();
         t64.insert(t64.end(), c.ooffs.begin(), c.ooffs.end());
         t32.insert(t32.end(), c.ptab.begin(), c.ptab.end());
+        sp.dims = t32.size();  // lefts then rights, (rows, cols) each
+        t32.insert(t32.end(), c.ldims.begin(), c.ldims.end());
+        t32.insert(t32.end(), c.rdims.begin(), c.rdims.end());
         spans.push_back(sp);
         fftk_job j{};
-        j.H = kv.first.first;
-        j.W = kv.first.second;
+        j.H = c.H;
+        j.W = c.W;
         j.nl = (uint32_t)c.loffs.size();
         j.nr = (uint32_t)c.roffs.size();
         j.pairs = (uint32_t)c.ooffs.size();
         plan->jobs.push_back(j);
-        // work estimate: FFT points transformed
-        cost.push_back((double)(j.nl + j.nr + 2.0 * j.pairs) * 4.0 * j.H * j.W);
+        // work estimate: FFT points transformed (padded lengths are proportional)
+        cost.push_back(c.work);
     }
     uint64_t* d64 = static_cast<uint64_t*>(plan_alloc(plan.get(), t64.size() * sizeof(uint64_t)));
     uint32_t* d32 = static_cast<uint32_t*>(plan_alloc(plan.get(), t32.size() * sizeof(uint32_t)));
@@ -171,6 +196,7 @@ xc_batch_plan* fft_plan_create(const xc_pair* pairs, uint32_t num_pairs, uint32_
         plan->jobs[c].moff = d64 + spans[c].moff;
         plan->jobs[c].ooff = d64 + spans[c].ooff;
         plan->jobs[c].ptab = d32 + spans[c].ptab;
+        plan->jobs[c].mdims = d32 + spans[c].dims;
     }
     // Classes, largest first, to the least loaded branch.
     const uint32_t nb = (uint32_t)std::max<size_t>(1, std::min<size_t>(branches, plan->jobs.size()));

@@ -342,10 +342,13 @@ struct rows_block {
 
 // rows_fwd: block (matrix, group of 2T rows); matrices 0..nl-1 are the left ones (into SL),
 // the rest the right ones (into SR).  Matrix i starts at element moff[i]
-// of its input when the offset table is given (uneven batches), else at i * H * W.
+// of its input when the offset table is given (uneven batches), else at i * H * W.  With
+// mdims (uneven batches), matrix i is mdims[i].x x mdims[i].y, at most H x W (the padded
+// lengths are those of H x W); S keeps a column stride of H.
 template <typename Tin, int LOGPC, int NT = rows_block<LOGPC, false>::NT>
 __global__ __launch_bounds__(NT) void rows_fwd_kernel(const Tin* __restrict__ XL, const Tin* __restrict__ XR,
-                                                       const uint64_t* __restrict__ moff, uint32_t nl, uint32_t H,
+                                                       const uint64_t* __restrict__ moff,
+                                                       const uint2* __restrict__ mdims, uint32_t nl, uint32_t H,
                                                        uint32_t W, int logT, uint32_t groups,
                                                        double2* __restrict__ SL, double2* __restrict__ SR,
                                                        const double2* __restrict__ tw) {
@@ -359,6 +362,13 @@ __global__ __launch_bounds__(NT) void rows_fwd_kernel(const Tin* __restrict__ XL
     const uint32_t mat = left ? matg : matg - nl;
     const int Pc = 1 << logPc, T = 1 << logT;
     const uint32_t r0 = g << (logT + 1);
+    uint32_t Hm = H, Wm = W;
+    if (mdims) {
+        const uint2 d = mdims[matg];
+        Hm = d.x;
+        Wm = d.y;
+    }
+    if (r0 >= Hm) return;  // the whole block: a row group beyond a smaller matrix of the job
     const Tin* Xm = (left ? XL : XR) + (moff ? moff[matg] : (size_t)mat * H * W);
     double2* S = left ? SL : SR;
     if constexpr (regs_of<LOGPC>::ON) {
@@ -372,20 +382,20 @@ __global__ __launch_bounds__(NT) void rows_fwd_kernel(const Tin* __restrict__ XL
         // Branch-free loads from clamped (in-bounds) addresses, all issued before the first
         // value is used; out-of-range positions are zeroed afterwards.  (Guarded loads make
         // the compiler wait on memory inside each guard.)
-        const bool oka = a < H, okb = b < H;
-        const Tin* xa = Xm + (oka ? (size_t)a * W : 0);
-        const Tin* xb = Xm + (okb ? (size_t)b * W : 0);
+        const bool oka = a < Hm, okb = b < Hm;
+        const Tin* xa = Xm + (oka ? (size_t)a * Wm : 0);
+        const Tin* xb = Xm + (okb ? (size_t)b * Wm : 0);
         Tin va[8], vb[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const uint32_t c = i + Q * r, cc = c < W ? c : W - 1;
+            const uint32_t c = i + Q * r, cc = c < Wm ? c : Wm - 1;
             va[r] = xa[cc];
             vb[r] = xb[cc];
         }
         double2 u[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            const bool in = i + Q * r < (int)W;
+            const bool in = i + Q * r < (int)Wm;
             u[r] = make_double2(oka && in ? (double)va[r] : 0.0, okb && in ? (double)vb[r] : 0.0);
         }
         fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), rr, t - t0, i);
@@ -404,9 +414,9 @@ __global__ __launch_bounds__(NT) void rows_fwd_kernel(const Tin* __restrict__ XL
             const int idx = threadIdx.x + kFT * q;
             const uint32_t t = idx >> logPc, c = idx & (Pc - 1);
             const uint32_t a = r0 + 2 * t, b = a + 1;
-            const size_t cc = c < W ? c : W - 1;
-            va[q] = Xm[(size_t)(a < H ? a : H - 1) * W + cc];
-            vb[q] = Xm[(size_t)(b < H ? b : H - 1) * W + cc];
+            const size_t cc = c < Wm ? c : Wm - 1;
+            va[q] = Xm[(size_t)(a < Hm ? a : Hm - 1) * Wm + cc];
+            vb[q] = Xm[(size_t)(b < Hm ? b : Hm - 1) * Wm + cc];
         }
 #pragma unroll
         for (int q = 0; q < NI; ++q) {
@@ -414,8 +424,8 @@ __global__ __launch_bounds__(NT) void rows_fwd_kernel(const Tin* __restrict__ XL
             if (idx >= n_el) continue;
             const uint32_t t = idx >> logPc, c = idx & (Pc - 1);
             const uint32_t a = r0 + 2 * t, b = a + 1;
-            const bool in = c < W;
-            buf[at<LOGPC>(t, c)] = make_double2(a < H && in ? (double)va[q] : 0.0, b < H && in ? (double)vb[q] : 0.0);
+            const bool in = c < Wm;
+            buf[at<LOGPC>(t, c)] = make_double2(a < Hm && in ? (double)va[q] : 0.0, b < Hm && in ? (double)vb[q] : 0.0);
         }
         __syncthreads();
         fft_slab<LOGPC>(buf, T, sr);
@@ -426,7 +436,7 @@ __global__ __launch_bounds__(NT) void rows_fwd_kernel(const Tin* __restrict__ XL
     for (int idx = threadIdx.x; idx < (int)Kc * rows2; idx += NT) {
         const int k = idx >> (logT + 1), rr = idx & (rows2 - 1);
         const uint32_t r = r0 + rr;
-        if (r >= H) continue;
+        if (r >= Hm) continue;
         const int t = rr >> 1;
         const double2 zk = buf[at<LOGPC>(t, k)];
         const double2 zn = cconj(buf[at<LOGPC>(t, (Pc - k) & (Pc - 1))]);
@@ -442,7 +452,9 @@ __global__ __launch_bounds__(NT) void rows_fwd_kernel(const Tin* __restrict__ XL
 }
 
 // cols: block (item, group of C columns k).  mode 0: item = left matrix l, writes its full
-// column spectra Lhat[l][k][rho] (the FFT along rho of SL[l][k][0..H), zero padded).
+// column spectra Lhat[l][k][rho] (the FFT along rho of SL[l][k][0..H), zero padded).  With
+// mdims (uneven batches) a matrix has mdims[i].x rows of its own (i = l, or nl + r for the
+// right of a pair); S keeps a column stride of H.
 // mode 1: item = pair q0 + item (left l, right r from the family's order, or from the pair
 // table of an uneven batch): X = the same FFT of SR[r][k], then a
 // second FFT of conj(Lhat) * X, conjugated: Z[item][k][rho] = Pr * IFFT_rho(Lhat * conj(X)).
@@ -452,7 +464,8 @@ __global__ __launch_bounds__(NT) void rows_fwd_kernel(const Tin* __restrict__ XL
 // compiler spills about 4 registers at 512 points), which measured faster than 3 waves.
 template <int LOGPR>
 __global__ __launch_bounds__(kFT, LOGPR <= 9 ? 4 : 1) void cols_kernel(const double2* __restrict__ S, double2* __restrict__ Lhat,
-                                                   double2* __restrict__ Z, const uint2* __restrict__ ptab, int mode,
+                                                   double2* __restrict__ Z, const uint2* __restrict__ ptab,
+                                                   const uint2* __restrict__ mdims, uint32_t nl, int mode,
                                                    int family, uint32_t n, uint32_t m, uint32_t q0, uint32_t items,
                                                    uint32_t H, uint32_t Kc, int logC, const double2* __restrict__ tw) {
     constexpr int logPr = LOGPR;
@@ -470,6 +483,7 @@ __global__ __launch_bounds__(kFT, LOGPR <= 9 ? 4 : 1) void cols_kernel(const dou
             pair_of(family, n, m, q0 + item, l, r);
         }
     }
+    const uint32_t Hq = mdims ? mdims[mode == 1 ? nl + r : l].x : H;  // rows of this item's matrices
     const double2* Sm = S + (size_t)r * Kc * H;
     double2* Ll = Lhat + (size_t)l * Kc * Pr;
     if constexpr (regs_of<LOGPR>::ON) {
@@ -487,7 +501,7 @@ __global__ __launch_bounds__(kFT, LOGPR <= 9 ? 4 : 1) void cols_kernel(const dou
         for (int r = 0; r < 8; ++r) {
             const uint32_t rho = i + Q * r;
             u[r] = make_double2(0.0, 0.0);
-            if (ok && rho < H) u[r] = Sm[(size_t)k * H + rho];
+            if (ok && rho < Hq) u[r] = Sm[(size_t)k * H + rho];
         }
         double2* lk = Ll + (size_t)k * Pr;
         // The left spectrum is loaded together with the column (its latency overlaps the
@@ -530,14 +544,14 @@ __global__ __launch_bounds__(kFT, LOGPR <= 9 ? 4 : 1) void cols_kernel(const dou
         const int idx = threadIdx.x + kFT * q;
         const uint32_t k = k0 + (idx >> logPr), rho = idx & (Pr - 1);
         const uint32_t kk = k < Kc ? k : Kc - 1;
-        xs[q] = Sm[(size_t)kk * H + (rho < H ? rho : H - 1)];
+        xs[q] = Sm[(size_t)kk * H + (rho < Hq ? rho : Hq - 1)];
         lh[q] = mode == 1 ? Ll[(size_t)kk * Pr + rho] : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
         const int idx = threadIdx.x + kFT * q;
         const uint32_t k = k0 + (idx >> logPr), rho = idx & (Pr - 1);
-        buf[at<LOGPR>(idx >> logPr, rho)] = (k < Kc && rho < H) ? xs[q] : make_double2(0.0, 0.0);
+        buf[at<LOGPR>(idx >> logPr, rho)] = (k < Kc && rho < Hq) ? xs[q] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_slab<LOGPR>(buf, C, sr);
@@ -579,14 +593,17 @@ __device__ __forceinline__ int out_row(uint32_t rho, uint32_t H, uint32_t Pr) {
 }
 
 // rows_inv: block (pair item, group of 2T consecutive circular rows rho0 = 2T*g ...), so that
-// each thread's two rows are adjacent in Z and a group's rows are one aligned run.  Row rho
+// each thread's two rows are adjacent in Z and a group's rows are one aligned run.  With
+// mdims (uneven batches) the pair's size is that of its right matrix nl + ptab[q].y.  Row rho
 // goes to output row out_row(rho); output column ox = dx + W - 1 takes circular column
 // dx mod Pc.  Pair q's result starts at ooff[q] when the table is given, else at q times the
 // result size.
 template <typename Tout, int LOGPC, int NT = rows_block<LOGPC, true>::NT>
 __global__ __launch_bounds__(NT) void rows_inv_kernel(const double2* __restrict__ Z, Tout* __restrict__ O,
-                                                       const uint64_t* __restrict__ ooff, uint32_t q0, uint32_t H,
-                                                       uint32_t W, int logPr, int logT, uint32_t groups, double scale,
+                                                       const uint64_t* __restrict__ ooff,
+                                                       const uint2* __restrict__ ptab, const uint2* __restrict__ mdims,
+                                                       uint32_t nl, uint32_t q0, uint32_t H, uint32_t W, int logPr,
+                                                       int logT, uint32_t groups, double scale,
                                                        const double2* __restrict__ tw) {
     constexpr int logPc = LOGPC;
     extern __shared__ double2 buf[];
@@ -598,6 +615,11 @@ __global__ __launch_bounds__(NT) void rows_inv_kernel(const double2* __restrict_
     const uint32_t Pr = 1u << logPr;
     const int half = Pc / 2;
     const uint32_t Kc = (uint32_t)half + 1;
+    if (mdims) {  // uneven batch: this pair's own size (at most H x W)
+        const uint2 d = mdims[nl + ptab[q0 + item].y];
+        H = d.x;
+        W = d.y;
+    }
     const uint32_t SX = 2 * W - 1;
     const uint32_t rho0 = g << (logT + 1);
     const int rows2 = 2 << logT;
@@ -815,6 +837,7 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
     double2* Zw = static_cast<double2*>(ws.z);
     const uint32_t chunk = std::max<uint32_t>(1, std::min<uint32_t>(j.pairs, ws.zchunk));
     const uint2* ptab = reinterpret_cast<const uint2*>(j.ptab);
+    const uint2* mdims = reinterpret_cast<const uint2*>(j.mdims);
     auto scope = [&](const char* name) { return timed(name, timing ? s : nullptr, timing); };
     {
         auto t0 = scope("fftk_rows_fwd");
@@ -822,7 +845,7 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
             auto k = rows_fwd_kernel<Tio, decltype(LP)::value>;
             allow_lds(k, g.v1.lds);
             hipLaunchKernelGGL(k, dim3((j.nl + j.nr) * g.g1), dim3(rows_threads(g.logPc, false)), g.v1.lds, s, L, R, j.moff,
-                               j.nl, H, W,
+                               mdims, j.nl, H, W,
                                g.v1.logT, g.g1, SL, SR, tw);
         });
     }
@@ -830,8 +853,8 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
         with_logp(g.logPr, [&](auto LP) {
             auto k = cols_kernel<decltype(LP)::value>;
             allow_lds(k, g.vc.lds);
-            hipLaunchKernelGGL(k, dim3(items * g.cg), dim3(kFT), g.vc.lds, s, S, LH, Zout, ptab, mode, j.family, j.n,
-                               j.m, q0, items, H, g.Kc, g.vc.logT, tw);
+            hipLaunchKernelGGL(k, dim3(items * g.cg), dim3(kFT), g.vc.lds, s, S, LH, Zout, ptab, mdims, j.nl, mode,
+                               j.family, j.n, j.m, q0, items, H, g.Kc, g.vc.logT, tw);
         });
     };
     {
@@ -850,7 +873,7 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
             auto k = rows_inv_kernel<Tio, decltype(LP)::value>;
             allow_lds(k, g.v3.lds);
             hipLaunchKernelGGL(k, dim3(nq * g.g3), dim3(rows_threads(g.logPc, true)), g.v3.lds, s, (const double2*)Zw, O,
-                               j.ooff, q0, H, W,
+                               j.ooff, ptab, mdims, j.nl, q0, H, W,
                                g.logPr, g.v3.logT, g.g3, scale, tw);
         });
     }

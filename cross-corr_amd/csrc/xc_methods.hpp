@@ -69,6 +69,9 @@ struct fftk_job {
     const uint64_t* moff = nullptr;
     const uint32_t* ptab = nullptr;
     const uint64_t* ooff = nullptr;
+    // Uneven batches: (rows, cols) of every matrix (lefts, then rights), each at most H x W
+    // and with the padded lengths of H x W; null when every matrix is H x W.
+    const uint32_t* mdims = nullptr;
 };
 struct fftk_ws_sizes {
     size_t sr, sl, lh, z;

@@ -226,3 +226,28 @@ def test_batch_fft_plan_branch_counts_agree():
             np.testing.assert_array_equal(got, ref)
     with pytest.raises(x.XcError):
         x.BatchPlan(pairs, "fft64", tile=1, branches=9)
+
+
+def test_batch_fft_plan_ragged_class_of_one_padded_size():
+    """Pairs of different sizes with the same padded lengths (256 x 128) share one FFT job with
+    a per-matrix size table; one left region is also read as two different sizes."""
+    rng = np.random.default_rng(21)
+    shapes = [(100, 37), (90, 50), (65, 64), (128, 33), (70, 40), (10, 10), (12, 10)]
+    lf = rng.random(200 * 200).astype(np.float32)
+    rf = (rng.random(200 * 200) * 2 - 1).astype(np.float32)
+    pairs, ooff = [], 0
+    roff = 0
+    for i, (r, c) in enumerate(shapes):
+        lo = 0 if (r, c) in ((10, 10), (12, 10)) else 1000 * i  # (10,10) and (12,10): one class, one left region
+        pairs.append((lo, roff, ooff, r, c))
+        roff += r * c
+        ooff += (2 * r - 1) * (2 * c - 1)
+    x = xcorr()
+    plan = x.BatchPlan(pairs, "fft64", tile=1)
+    out = torch.full((ooff,), float("nan"), device="cuda")
+    plan.run(dev(lf), dev(rf), out, use_graph=True)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    plan.close()
+    assert not np.isnan(got).any()
+    batch_oracle_check(lf, rf, pairs, got, 1e-6)
