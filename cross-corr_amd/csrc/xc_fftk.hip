@@ -460,10 +460,8 @@ __global__ __launch_bounds__(NT) void rows_fwd_kernel(const Tin* __restrict__ XL
 // second FFT of conj(Lhat) * X, conjugated: Z[item][k][rho] = Pr * IFFT_rho(Lhat * conj(X)).
 // Blocks are ordered column-group major through the XCD remap, so the pairs that read the
 // same Lhat columns run on one XCD and find them in its L2.
-// Up to 512 points the kernel is held to 128 VGPRs (4 waves per SIMD, as its LDS allows; the
-// compiler spills about 4 registers at 512 points), which measured faster than 3 waves.
 template <int LOGPR>
-__global__ __launch_bounds__(kFT, LOGPR <= 9 ? 4 : 1) void cols_kernel(const double2* __restrict__ S, double2* __restrict__ Lhat,
+__global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S, double2* __restrict__ Lhat,
                                                    double2* __restrict__ Z, const uint2* __restrict__ ptab,
                                                    const uint2* __restrict__ mdims, uint32_t nl, int mode,
                                                    int family, uint32_t n, uint32_t m, uint32_t q0, uint32_t items,
