@@ -12,9 +12,13 @@
 Every method is timed with HIP events on the stream it runs on (W warm-up calls, then K
 timed calls; K = 1 when one call takes over 250 ms) and its max normalised error
 (|x - x64| / corr(|L|,|R|)) is measured against the FP64 oracle on a subset of results.
-Prints one JSON line per configuration (and progress lines on stderr).
+Beside them, the oracle's FP32 restatement of the reference's cpu_cross_corr_* (the
+reference's CPU path) is timed on a bounded sample of each workload on the host cores
+(`cpu_baseline`: one thread and up to 16 threads; samples are stated, larger workloads are
+extrapolated by their MACs and marked so).  Prints one JSON line per configuration (and
+progress lines on stderr).
 
-    python bench_configs.py [--configs c1,c3,c4,c5] [--steps 10] [--warmup 2]
+    python tests/bench_configs.py [--configs c1,c3,c4,c5] [--steps 10] [--warmup 2]
 """
 import argparse
 import json
@@ -22,7 +26,7 @@ import pathlib
 import sys
 import time
 
-REPO = pathlib.Path(__file__).resolve().parent
+REPO = pathlib.Path(__file__).resolve().parent.parent
 sys.path.insert(0, str(REPO / "tests"))
 
 import numpy as np  # noqa: E402
@@ -124,6 +128,39 @@ def main():
             log(f"{name} {m}: {ms:.4f} ms" + (f" err {r['max_norm_err']:.2e}" if "max_norm_err" in r else ""))
         return res
 
+    import ctypes
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    from xc_testutil import oracle
+
+    ncores = max(1, min(os.cpu_count() or 1, 16))  # the host share of one GPU
+
+    def cpu_family(family, left, right, k0, k1, threads):
+        """Seconds the oracle's FP32 restatement of cpu_cross_corr_<family> takes for results
+        [k0, k1) of a regular family on `threads` host threads."""
+        lib = oracle()
+        fam = x.FAMILIES[family]
+        n, r, c = left.shape
+        m = right.shape[0]
+        buf = np.zeros((k1 - k0, 2 * r - 1, 2 * c - 1), np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        lc, rc = np.ascontiguousarray(left, np.float32), np.ascontiguousarray(right, np.float32)
+        t0 = time.perf_counter()
+        lib.xco_family_range_f32(fam, lc.ctypes.data_as(fp), rc.ctypes.data_as(fp), buf.ctypes.data_as(fp),
+                                 r, c, n, m, k0, k1, threads)
+        return time.perf_counter() - t0
+
+    def cpu_baseline(family, left, right, pairs, sample):
+        """pairs/s of the CPU path on one thread and on ncores threads, timed on `sample`
+        results (linear in pairs beyond them)."""
+        t1 = cpu_family(family, left, right, 0, min(sample, pairs), 1) / min(sample, pairs)
+        kall = min(max(ncores, sample), pairs)
+        tn = cpu_family(family, left, right, 0, kall, ncores) / kall
+        return {"value_1_thread": 1.0 / t1, "value_all_cores": 1.0 / tn, "unit": "pairs/s", "cores": ncores,
+                "kind": "port", "sample": f"{min(sample, pairs)} results on 1 thread, {kall} on {ncores} threads"
+                + ("" if kall >= pairs else f"; {pairs} extrapolated linearly")}
+
     def summary(name, workload, res, pairs, extra=None):
         auto = res.get("auto", {})
         fp32 = res.get("fp32", {})
@@ -145,7 +182,8 @@ def main():
         right = seeded((1, 128, 128), "u01", 2, np.float32)
         res = measure_family("c1", "one_to_one", left, right, ["fp32", "bf16x3", "fft64", "fft64k", "auto"], [0], 1)
         summary("c1", "one_to_one 128x128 f32 U[0,1)", res, 1,
-                {"auto_selected": auto_name("one_to_one", 128, 128, 1, 1)})
+                {"auto_selected": auto_name("one_to_one", 128, 128, 1, 1),
+                 "cpu_baseline": cpu_baseline("one_to_one", left, right, 1, 1)})
 
     if "c3" in cfgs:
         left = seeded((16, 64, 64), "normal", 3, np.float32)
@@ -156,7 +194,8 @@ def main():
         res64 = measure_family("c3-f64", "n_to_m", left.astype(np.float64), right.astype(np.float64),
                                ["fp32", "fft64", "fft64k"], chk, 1024)
         summary("c3", "n_to_m 16x64 of 64x64 f32 N(0,1); double data under 'double'", res, 1024,
-                {"double": res64, "auto_selected": auto_name("n_to_m", 64, 64, 16, 64)})
+                {"double": res64, "auto_selected": auto_name("n_to_m", 64, 64, 16, 64),
+                 "cpu_baseline": cpu_baseline("n_to_m", left, right, 1024, 16)})
 
     if "c4" in cfgs:
         lf, rf, pairs, olen, sizes = uneven_batch(128, 2024, lo=16, hi=512, per_group=4, dist="u11")
@@ -219,6 +258,24 @@ def main():
             nm = auto_name("one_to_many", nsz, nsz, 1, 4)[0]
             picks[nm] = picks.get(nm, 0) + 1
         fp32 = res["per_class_fp32"]["ms"]
+        # CPU path on a sample: 16 pairs spread over the sizes up to 200 (one thread each),
+        # extrapolated to the 512 pairs by their MACs (the largest pairs alone would take minutes)
+        small = sorted((i for i in range(len(pairs)) if pairs[i][3] <= 200), key=lambda i: pairs[i][3])
+        samp = [small[int(v)] for v in np.linspace(0, len(small) - 1, 16)]
+
+        def cpu_pair(i):
+            lo, ro, _, r, c = pairs[i]
+            return cpu_family("one_to_one", lf[lo:lo + r * c].reshape(1, r, c), rf[ro:ro + r * c].reshape(1, r, c),
+                              0, 1, 1)
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(ncores) as ex:
+            list(ex.map(cpu_pair, samp))
+        wall = time.perf_counter() - t0
+        macs = lambda i: float(pairs[i][3]) ** 2 * float(pairs[i][4]) ** 2
+        frac = sum(macs(i) for i in samp) / sum(macs(i) for i in range(len(pairs)))
+        cpu = {"value_all_cores": len(pairs) * frac / wall, "unit": "pairs/s", "cores": ncores, "kind": "port",
+               "sample": f"16 pairs of sizes {pairs[samp[0]][3]}..{pairs[samp[-1]][3]} ({frac:.2e} of the MACs) "
+                         f"in {wall:.1f} s on {ncores} threads; 512 pairs extrapolated by MACs"}
         line = {"config": "c4", "workload": "uneven n_to_mn, 128 groups of 1+4 square matrices, sizes "
                 "round(16*32^U) seed 2024, f32 U[-1,1)", "pairs": 512, "sizes_min_max": [min(sizes), max(sizes)],
                 "methods": res, "auto_picks_per_group": picks,
@@ -229,7 +286,8 @@ def main():
                 "batch_graph_vs_per_class_bf16x3": res["per_class_bf16x3"]["ms"] / res["batch_bf16x3_graph"]["ms"],
                 "batch_graph_vs_fp32_direct": fp32 / res["batch_bf16x3_graph"]["ms"],
                 "fft_batch_graph_vs_fp32_direct": fp32 / res["batch_fft64k_graph"]["ms"],
-                "fft_batch_graph_vs_per_class_fft64k": res["per_class_fft64k"]["ms"] / res["batch_fft64k_graph"]["ms"]}
+                "fft_batch_graph_vs_per_class_fft64k": res["per_class_fft64k"]["ms"] / res["batch_fft64k_graph"]["ms"],
+                "cpu_baseline": cpu}
         print(json.dumps(line), flush=True)
 
     if "c5" in cfgs:
@@ -246,7 +304,9 @@ def main():
             rows.append({"N": nsz, "methods": res, "fastest_measured": best, "auto_selected": pick, "selected_by": src,
                          "auto_regret": res["auto"]["ms"] / timed_ok[best] if "ms" in res.get("auto", {}) else None,
                          "definition_over_fft": res["bf16x3"]["ms"] / min(res["fft64"]["ms"], res["fft64k"]["ms"])
-                         if all("ms" in res.get(m, {}) for m in ("bf16x3", "fft64", "fft64k")) else None})
+                         if all("ms" in res.get(m, {}) for m in ("bf16x3", "fft64", "fft64k")) else None,
+                         # N >= 512: over a minute per pair on one thread; not timed
+                         "cpu_baseline": cpu_baseline("one_to_many", left, right, 128, 1) if nsz <= 256 else None})
         print(json.dumps({"config": "c5", "workload": "one_to_many 1x128 f32 U[0,1)", "sweep": rows}), flush=True)
 
 
