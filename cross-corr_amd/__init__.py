@@ -19,9 +19,9 @@ LIB_PATH = PKG_DIR / "libxcorr.so"
 ONE_TO_ONE, ONE_TO_MANY, N_TO_MN, N_TO_M = 0, 1, 2, 3
 FAMILIES = {"one_to_one": ONE_TO_ONE, "one_to_many": ONE_TO_MANY, "n_to_mn": N_TO_MN, "n_to_m": N_TO_M}
 
-PREC_AUTO, PREC_FP32, PREC_BF16X3, PREC_BF16, PREC_FP16, PREC_FFT64, PREC_FFT32 = range(7)
+PREC_AUTO, PREC_FP32, PREC_BF16X3, PREC_BF16, PREC_FP16, PREC_FFT64, PREC_FFT32, PREC_FP16X3 = range(8)
 PRECISIONS = {"auto": PREC_AUTO, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3, "bf16": PREC_BF16,
-              "fp16": PREC_FP16, "fft64": PREC_FFT64, "fft32": PREC_FFT32}
+              "fp16": PREC_FP16, "fft64": PREC_FFT64, "fft32": PREC_FFT32, "fp16x3": PREC_FP16X3}
 
 STATUS = {0: "XC_OK", 1: "XC_ERR_INVALID_ARGUMENT", 2: "XC_ERR_HIP", 3: "XC_ERR_FFT",
           4: "XC_ERR_UNSUPPORTED", 5: "XC_ERR_NO_MEMORY"}

@@ -332,6 +332,7 @@ std::vector<alg_spec> make_algorithms() {
     for (int f = 0; f < 4; ++f) {
         const std::string fam = kFamilyNames[f];
         add("mfma_bf16x3_" + fam, f, method::mfma, XC_PREC_BF16X3, {tile, border});
+        add("mfma_fp16x3_" + fam, f, method::mfma, XC_PREC_FP16X3, {tile, border});
         add("mfma_bf16_" + fam, f, method::mfma, XC_PREC_BF16, {tile});
         add("mfma_fp16_" + fam, f, method::mfma, XC_PREC_FP16, {tile});
         add("auto_" + fam, f, method::autosel, XC_PREC_AUTO, {});

@@ -88,7 +88,8 @@ xc_status xc_cross_corr_f32(xc_family family, const float* left, const float* ri
         check_family(family, rows, cols, n, m, left, right, out);
         xc_options o{};
         if (opt) o = *opt;
-        if (o.precision > XC_PREC_FFT32) xc::fail(XC_ERR_INVALID_ARGUMENT, "unknown precision " + std::to_string(o.precision));
+        if (o.precision > XC_PREC_FP16X3)
+            xc::fail(XC_ERR_INVALID_ARGUMENT, "unknown precision " + std::to_string(o.precision));
         if (o.border > 512) xc::fail(XC_ERR_INVALID_ARGUMENT, "border threshold above 512");
         xc::family_args a{family, rows, cols, n, m, (uint32_t)xc::num_results(family, n, m)};
         xc::run_family_f32(a, left, right, out, o, (hipStream_t)stream);

@@ -56,6 +56,7 @@ void run_family_f32(const family_args& a, const float* L, const float* R, float*
         launch_direct_f32(a, L, R, O, opt.split, s);
         return;
     case XC_PREC_BF16X3:
+    case XC_PREC_FP16X3:
     case XC_PREC_BF16:
     case XC_PREC_FP16:
         launch_mfma_f32(a, L, R, O, (int)opt.precision, opt.border, opt.tile, s);

@@ -57,6 +57,16 @@ struct split_traits<XC_PREC_BF16> {
     }
 };
 template <>
+struct split_traits<XC_PREC_FP16X3> {
+    static constexpr int planes = 2;
+    __device__ static void split(float x, uint16_t& hi, uint16_t& lo) {
+        _Float16 h = (_Float16)x;
+        _Float16 l = (_Float16)(x - (float)h);
+        hi = __builtin_bit_cast(uint16_t, h);
+        lo = __builtin_bit_cast(uint16_t, l);
+    }
+};
+template <>
 struct split_traits<XC_PREC_FP16> {
     static constexpr int planes = 1;
     __device__ static void split(float x, uint16_t& hi, uint16_t& lo) {
@@ -67,7 +77,7 @@ struct split_traits<XC_PREC_FP16> {
 
 template <int KIND>
 __device__ __forceinline__ f32x16 mma(const uint4& a, const uint4& b, f32x16 c) {
-    if constexpr (KIND == XC_PREC_FP16)
+    if constexpr (KIND == XC_PREC_FP16 || KIND == XC_PREC_FP16X3)
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), c,
                                                       0, 0, 0);
     else
@@ -557,8 +567,10 @@ void launch(const family_args& a, const float* L, const float* R, float* O, uint
         attr_set = true;
     }
     dim3 grid((uint32_t)nblocks), block(256);
-    record_launch(KIND == XC_PREC_BF16X3 ? "mfma_kernel<bf16x3>" : (KIND == XC_PREC_BF16 ? "mfma_kernel<bf16>"
-                                                                                          : "mfma_kernel<fp16>"),
+    record_launch(KIND == XC_PREC_BF16X3   ? "mfma_kernel<bf16x3>"
+                  : KIND == XC_PREC_FP16X3 ? "mfma_kernel<fp16x3>"
+                  : KIND == XC_PREC_BF16   ? "mfma_kernel<bf16>"
+                                           : "mfma_kernel<fp16>",
                   grid, block, G::BYTES, a.rows, a.cols, a.pairs, 1);
     timed tm("mfma_kernel", s);
     hipLaunchKernelGGL((mfma_kernel<KIND, WX, WY, KC, FLUSH, KAHAN, OCC>), grid, block, G::BYTES, s, L, R, O, a.family, a.rows, a.cols, a.n,
@@ -626,7 +638,8 @@ __global__ __launch_bounds__(256, kBOCC) void batch_kernel(const batch_item* __r
         float* Op = O + pd.ooff;
         if (bi.kind == 0) {
             mfma_tile<KIND, kBWX, kBWY, kBKC, kBFLUSH, kBKAHAN, kBOCC>(Lp, Rp, Op, H, W, bi.tx, bi.b,
-                                                                      KIND == XC_PREC_BF16X3 ? thresh : 0, smem);
+                                                                      split_traits<KIND>::planes == 2 ? thresh : 0,
+                                                                      smem);
         } else {
             const uint32_t* start = tables + pd.table_off;
             const uint32_t total = start[pd.nh] * 4;
@@ -680,12 +693,14 @@ const void* batch_kernel_ptr(int kind) {
     case XC_PREC_BF16X3: return reinterpret_cast<const void*>(&batch_kernel<XC_PREC_BF16X3>);
     case XC_PREC_BF16: return reinterpret_cast<const void*>(&batch_kernel<XC_PREC_BF16>);
     case XC_PREC_FP16: return reinterpret_cast<const void*>(&batch_kernel<XC_PREC_FP16>);
-    default: fail(XC_ERR_INVALID_ARGUMENT, "the batch kernel supports bf16x3, bf16 and fp16");
+    case XC_PREC_FP16X3: return reinterpret_cast<const void*>(&batch_kernel<XC_PREC_FP16X3>);
+    default: fail(XC_ERR_INVALID_ARGUMENT, "the batch kernel supports bf16x3, fp16x3, bf16 and fp16");
     }
 }
 
 uint32_t batch_lds_bytes(int kind) {
-    return kind == XC_PREC_BF16X3 ? geom<kBWX, kBWY, 2, kBKC>::BYTES : geom<kBWX, kBWY, 1, kBKC>::BYTES;
+    return kind == XC_PREC_BF16X3 || kind == XC_PREC_FP16X3 ? geom<kBWX, kBWY, 2, kBKC>::BYTES
+                                                             : geom<kBWX, kBWY, 1, kBKC>::BYTES;
 }
 
 uint32_t batch_blocks_per_cu() { return kBOCC; }
@@ -722,6 +737,12 @@ void launch_mfma_f32(const family_args& a, const float* L, const float* R, float
     case XC_PREC_BF16X3: {
         const uint32_t thresh = border ? border : kDefaultBorder;
         launch_tile<XC_PREC_BF16X3>(a, L, R, O, tile, thresh, s);
+        launch_frame_f32(a, L, R, O, thresh, s);
+        break;
+    }
+    case XC_PREC_FP16X3: {
+        const uint32_t thresh = border ? border : kDefaultBorder;
+        launch_tile<XC_PREC_FP16X3>(a, L, R, O, tile, thresh, s);
         launch_frame_f32(a, L, R, O, thresh, s);
         break;
     }

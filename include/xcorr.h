@@ -60,6 +60,10 @@ typedef enum xc_family {
  *                   the border threshold are recomputed exactly in FP32
  *  XC_PREC_BF16     MFMA, single bf16 product (fast, ~1e-3 relative error)
  *  XC_PREC_FP16     MFMA, single fp16 product (inputs must fit fp16 range)
+ *  XC_PREC_FP16X3   MFMA, each operand split into fp16 hi + lo (hi*hi + hi*lo + lo*hi in
+ *                   FP32, border outputs exact in FP32 as for bf16x3); 22 significant bits
+ *                   per operand, inputs must fit fp16 range (|x| < 65504; the lo part of
+ *                   |x| < 2^-3 loses bits to fp16 subnormals)
  *  XC_PREC_FFT64    FFT method computed in FP64, results in FP32; tile 0: rocFFT D2Z/Z2D,
  *                   tile 1: this library's fused FP64 FFT kernels (rows, cols <= 2048)
  *  XC_PREC_FFT32    FFT method in FP32 (rocFFT R2C/C2R); large error at low-overlap outputs
@@ -71,7 +75,8 @@ typedef enum xc_precision {
     XC_PREC_BF16 = 3,
     XC_PREC_FP16 = 4,
     XC_PREC_FFT64 = 5,
-    XC_PREC_FFT32 = 6
+    XC_PREC_FFT32 = 6,
+    XC_PREC_FP16X3 = 7
 } xc_precision;
 
 /* Optional tuning; zero-initialise and set what you need (0 = library default). */

@@ -184,7 +184,8 @@ def test_runner_invocation_shape_cpu(tmp_path):
 
 # ------------------------------------------------------------------ GPU
 GPU_NAMES = [n for n in REFERENCE_NAMES if not n.startswith("cpu_")] + [
-    f"{m}_{f}" for m in ("mfma_bf16x3", "auto") for f in ("one_to_one", "one_to_many", "n_to_mn", "n_to_m")]
+    f"{m}_{f}" for m in ("mfma_bf16x3", "mfma_fp16x3", "auto") for f in ("one_to_one", "one_to_many", "n_to_mn",
+                                                                            "n_to_m")]
 
 
 @pytest.mark.gpu

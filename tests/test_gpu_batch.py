@@ -16,10 +16,11 @@ def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-def test_batch_small_uneven_bf16x3():
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])
+def test_batch_small_uneven_bf16x3(prec):
     lf, rf, pairs, olen, sizes = uneven_batch(0, 3, sizes=[1, 2, 5, 16, 33, 64, 70, 100, 140, 31])
     out = torch.full((olen,), float("nan"), device="cuda")
-    xcorr().batch(pairs, dev(lf), dev(rf), out, precision="bf16x3")
+    xcorr().batch(pairs, dev(lf), dev(rf), out, precision=prec)
     torch.cuda.synchronize()
     assert not torch.isnan(out).any()
     batch_oracle_check(lf, rf, pairs, out.cpu().numpy(), 2e-6)

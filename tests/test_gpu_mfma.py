@@ -1,7 +1,7 @@
 """GPU parity of the MFMA implicit-GEMM kernels against the FP64 oracle.
 
 Bars (normalised error |x - x64| / corr(|L|,|R|)):
-  bf16x3 (with the exact FP32 border frame) <= 2e-6 everywhere;
+  bf16x3 and fp16x3 (with the exact FP32 border frame) <= 2e-6 everywhere;
   single bf16 <= 8e-3 and single fp16 <= 1e-3 (one rounding of each operand, products summed in FP32).
 """
 import numpy as np
@@ -51,20 +51,32 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])
 @pytest.mark.parametrize("family,rows,cols,n,m,dist", CASES)
-def test_bf16x3_seeded(family, rows, cols, n, m, dist):
+def test_bf16x3_seeded(family, rows, cols, n, m, dist, prec):
     left = seeded((n, rows, cols), dist, 500 + rows, np.float32)
     right = seeded((m, rows, cols), dist, 600 + cols, np.float32)
-    err, _, _ = errors(family, left, right, run(family, left, right, precision="bf16x3"))
+    err, _, _ = errors(family, left, right, run(family, left, right, precision=prec))
     assert err.max() <= 2e-6, err.max()
 
 
+def test_fp16x3_is_closer_than_bf16x3_on_unit_data():
+    """fp16 hi + lo keeps 22 significant bits of each operand against bf16's 16: on data of
+    order one (no fp16 range or subnormal limits) the split-fp16 result is the closer one."""
+    left = seeded((1, 96, 96), "u01", 71, np.float32)
+    right = seeded((4, 96, 96), "u01", 72, np.float32)
+    e_bf, _, _ = errors("one_to_many", left, right, run("one_to_many", left, right, precision="bf16x3", border=1))
+    e_fp, _, _ = errors("one_to_many", left, right, run("one_to_many", left, right, precision="fp16x3", border=1))
+    assert e_fp.max() < e_bf.max(), (e_fp.max(), e_bf.max())
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])
 @pytest.mark.parametrize("valid_name", sorted(FIXTURES))
-def test_bf16x3_golden(valid_name):
+def test_bf16x3_golden(valid_name, prec):
     left = load_csv(DATA / FIXTURES[valid_name][0]).astype(np.float32)
     right = load_csv(DATA / FIXTURES[valid_name][1]).astype(np.float32)
     fam = fixture_family(valid_name)
-    got = run(fam, left, right, precision="bf16x3")
+    got = run(fam, left, right, precision=prec)
     err, _, den = errors(fam, left, right, got)
     assert err.max() <= 2e-6
     assert norm_error(got, load_csv(DATA / valid_name), den)[0] <= 3e-6
