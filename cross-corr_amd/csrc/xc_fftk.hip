@@ -23,6 +23,7 @@
 // longer ones are transformed by a block in an LDS slab of up to 4096 values (padded LDS
 // index against bank conflicts).  Twiddles come from a 4096-entry table, read once per kernel
 // into registers; every HBM load of a kernel is issued before its first use.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -823,6 +824,19 @@ fftk_geom geom_of(uint32_t H, uint32_t W) {
     return g;
 }
 
+// Launch, with the kernel's own start and end recorded under `name` while kernel timing is on.
+template <typename K, typename... Args>
+void launch_timed(const char* name, bool timing, K kernel, dim3 grid, dim3 block, uint32_t lds, hipStream_t s,
+                  Args... args) {
+    if (timing && timing_enabled()) {
+        hipEvent_t a = nullptr, b = nullptr;
+        timing_launch_events(name, &a, &b);
+        hipExtLaunchKernelGGL(kernel, grid, block, lds, s, a, b, 0, args...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+    }
+}
+
 template <typename Tio>
 void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fftk_ws& ws, hipStream_t s,
                 bool timing) {
@@ -836,43 +850,31 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
     const uint32_t chunk = std::max<uint32_t>(1, std::min<uint32_t>(j.pairs, ws.zchunk));
     const uint2* ptab = reinterpret_cast<const uint2*>(j.ptab);
     const uint2* mdims = reinterpret_cast<const uint2*>(j.mdims);
-    auto scope = [&](const char* name) { return timed(name, timing ? s : nullptr, timing); };
-    {
-        auto t0 = scope("fftk_rows_fwd");
-        with_logp(g.logPc, [&](auto LP) {
-            auto k = rows_fwd_kernel<Tio, decltype(LP)::value>;
-            allow_lds(k, g.v1.lds);
-            hipLaunchKernelGGL(k, dim3((j.nl + j.nr) * g.g1), dim3(rows_threads(g.logPc, false)), g.v1.lds, s, L, R, j.moff,
-                               mdims, j.nl, H, W,
-                               g.v1.logT, g.g1, SL, SR, tw);
-        });
-    }
-    auto cols = [&](const double2* S, double2* Zout, int mode, uint32_t items, uint32_t q0) {
+    with_logp(g.logPc, [&](auto LP) {
+        auto k = rows_fwd_kernel<Tio, decltype(LP)::value>;
+        allow_lds(k, g.v1.lds);
+        launch_timed("fftk_rows_fwd", timing, k, dim3((j.nl + j.nr) * g.g1), dim3(rows_threads(g.logPc, false)),
+                     g.v1.lds, s, L, R, j.moff, mdims, j.nl, H, W, g.v1.logT, g.g1, SL, SR, tw);
+    });
+    auto cols = [&](const char* name, const double2* S, double2* Zout, int mode, uint32_t items, uint32_t q0) {
         with_logp(g.logPr, [&](auto LP) {
             auto k = cols_kernel<decltype(LP)::value>;
             allow_lds(k, g.vc.lds);
-            hipLaunchKernelGGL(k, dim3(items * g.cg), dim3(kFT), g.vc.lds, s, S, LH, Zout, ptab, mdims, j.nl, mode,
-                               j.family, j.n, j.m, q0, items, H, g.Kc, g.vc.logT, tw);
+            launch_timed(name, timing, k, dim3(items * g.cg), dim3(kFT), g.vc.lds, s, S, LH, Zout, ptab,
+                         mdims, j.nl, mode, j.family, j.n, j.m, q0, items, H, g.Kc, g.vc.logT, tw);
         });
     };
-    {
-        auto t1 = scope("fftk_cols_left");
-        cols(SL, nullptr, 0, j.nl, 0);
-    }
+    cols("fftk_cols_left", SL, nullptr, 0, j.nl, 0);
     const double scale = 1.0 / ((double)g.Pr * (double)g.Pc);
     for (uint32_t q0 = 0; q0 < j.pairs; q0 += chunk) {
         const uint32_t nq = std::min(chunk, j.pairs - q0);
-        {
-            auto t2 = scope("fftk_cols");
-            cols(SR, Zw, 1, nq, q0);
-        }
-        auto t3 = scope("fftk_rows_inv");
+        cols("fftk_cols", SR, Zw, 1, nq, q0);
         with_logp(g.logPc, [&](auto LP) {
             auto k = rows_inv_kernel<Tio, decltype(LP)::value>;
             allow_lds(k, g.v3.lds);
-            hipLaunchKernelGGL(k, dim3(nq * g.g3), dim3(rows_threads(g.logPc, true)), g.v3.lds, s, (const double2*)Zw, O,
-                               j.ooff, ptab, mdims, j.nl, q0, H, W,
-                               g.logPr, g.v3.logT, g.g3, scale, tw);
+            launch_timed("fftk_rows_inv", timing, k, dim3(nq * g.g3), dim3(rows_threads(g.logPc, true)), g.v3.lds, s,
+                         (const double2*)Zw, O, j.ooff, ptab, mdims, j.nl, q0, H, W, g.logPr,
+                         g.v3.logT, g.g3, scale, tw);
         });
     }
     XC_HIP(hipGetLastError());

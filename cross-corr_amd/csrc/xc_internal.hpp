@@ -37,6 +37,9 @@ bool timing_enabled();
 void timing_enable(bool on);
 std::string timing_report();  // "name count total_ms" lines; clears the records
 void timing_mark(const char* name, hipStream_t s, int phase);
+// Two new events, registered under `name`, for the caller to attach to one launch
+// (hipExtLaunchKernel's start and stop events).
+void timing_launch_events(const char* name, hipEvent_t* start, hipEvent_t* stop);
 struct timed {
     const char* name;
     hipStream_t s;

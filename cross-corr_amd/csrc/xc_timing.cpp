@@ -1,5 +1,7 @@
-// Kernel timing for measurement runs: HIP events recorded on each launch's own stream just
-// before and after the launch, collected per kernel name.  Off by default; not meant for
+// Kernel timing for measurement runs: HIP events on each launch's own stream, collected per
+// kernel name -- recorded just before and after the launch (`timed` scopes), or attached to
+// the kernel's own dispatch (timing_launch_events with hipExtLaunchKernel: the kernel's start
+// and end, without the event packets' own processing time).  Off by default; not meant for
 // launches under stream capture.
 #include <hip/hip_runtime.h>
 
@@ -41,6 +43,19 @@ void timing_mark(const char* name, hipStream_t s, int phase) {
     std::lock_guard<std::mutex> lock(g_mu);
     g_done.push_back(t_open);
     t_open = record{};
+}
+
+void timing_launch_events(const char* name, hipEvent_t* start, hipEvent_t* stop) {
+    record r{name, nullptr, nullptr};
+    XC_HIP(hipEventCreate(&r.a));
+    if (hipEventCreate(&r.b) != hipSuccess) {
+        (void)hipEventDestroy(r.a);
+        fail(XC_ERR_HIP, "event creation failed");
+    }
+    *start = r.a;
+    *stop = r.b;
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_done.push_back(r);
 }
 
 void timing_enable(bool on) { g_on.store(on); }
