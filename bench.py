@@ -38,7 +38,7 @@ MACS_PER_PAIR = ROWS * COLS * ROWS * COLS
 ERR_BAR = 2e-6
 HBM_GBPS = 8000.0
 TFLOPS = {"direct_kernel": 157.3, "mfma_kernel": 2500.0}  # FP32 VALU; dense BF16 MFMA
-METHODS = {"fp32": dict(precision="fp32"), "bf16x3": dict(precision="bf16x3"),
+METHODS = {"fp32": dict(precision="fp32"), "bf16x3": dict(precision="bf16x3"), "fp16x3": dict(precision="fp16x3"),
            "fft64": dict(precision="fft64", tile=0), "fft64k": dict(precision="fft64", tile=1),
            "auto": dict(precision="auto")}
 
@@ -108,7 +108,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--methods", default="fp32,bf16x3,fft64,fft64k,auto")
+    ap.add_argument("--methods", default="fp32,bf16x3,fp16x3,fft64,fft64k,auto")
     ap.add_argument("--cpu-pairs", type=int, default=3, help="pairs timed for the CPU baseline sample")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy 3 times (the known-size kernel that scales "

@@ -85,7 +85,7 @@ def main():
         prec, tile = meth
         return lambda: x.cross_corr(family, L, R, out=out, precision=prec, tile=tile)
 
-    METH = {"fp32": ("fp32", 0), "bf16x3": ("bf16x3", 0), "bf16": ("bf16", 0), "fp16": ("fp16", 0),
+    METH = {"fp32": ("fp32", 0), "bf16x3": ("bf16x3", 0), "fp16x3": ("fp16x3", 0), "bf16": ("bf16", 0), "fp16": ("fp16", 0),
             "fft64": ("fft64", 0), "fft64k": ("fft64", 1), "auto": ("auto", 0)}
 
     def measure_family(name, family, left, right, methods, check, pairs):
@@ -190,7 +190,7 @@ def main():
         right = seeded((64, 64, 64), "normal", 4, np.float32)
         chk = [0, 63, 64 * 7 + 5, 1023]
         res = measure_family("c3", "n_to_m", left, right,
-                             ["fp32", "bf16x3", "bf16", "fp16", "fft64", "fft64k", "auto"], chk, 1024)
+                             ["fp32", "bf16x3", "fp16x3", "bf16", "fp16", "fft64", "fft64k", "auto"], chk, 1024)
         res64 = measure_family("c3-f64", "n_to_m", left.astype(np.float64), right.astype(np.float64),
                                ["fp32", "fft64", "fft64k"], chk, 1024)
         summary("c3", "n_to_m 16x64 of 64x64 f32 N(0,1); double data under 'double'", res, 1024,
