@@ -18,11 +18,13 @@
 //              rows of Z are extended, inverse transformed, scaled by 1/(Pr*Pc) and written
 //              straight into the cropped (2H-1) x (2W-1) result (quadrant swap by index).
 //
-// All arithmetic is FP64, in radix-8/4/2 Stockham passes.  Transforms of up to 512 points
-// live in registers (one wave owns each, LDS only for the exchanges between passes);
-// longer ones are transformed by a block in an LDS slab of up to 4096 values (padded LDS
-// index against bank conflicts).  Twiddles come from a 4096-entry table, read once per kernel
-// into registers; every HBM load of a kernel is issued before its first use.
+// All arithmetic is FP64, in radix-8/4/2 Stockham passes.  Transforms of up to 2048 points
+// live in registers (a wave owns each up to 512 points, 2 or 4 waves of a block at 1024 and
+// 2048; LDS only for the exchanges between passes); 4096-point transforms are done by a block
+// in an LDS slab (padded LDS index against bank conflicts).  Twiddles come from a 4096-entry
+// table, read once per kernel into registers; every HBM load of a kernel is issued before its
+// first use.  Uneven batches run jobs whose matrices share the padded lengths but keep their
+// own sizes (a per-matrix size table).
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -51,7 +53,7 @@ __device__ __forceinline__ int at(int t, int j) {
 }
 
 // Slab (values per block) of a transform length: 2^11 values, or one transform of 4096.
-// Lengths of up to 512 points are transformed in registers (fft_regs), longer ones in slabs.
+// Lengths of up to 2048 points are transformed in registers (fft_regs), 4096 in a slab.
 template <int LOGP>
 struct slab_of {
     static constexpr int LOGS = LOGP <= 11 ? 11 : LOGP;
