@@ -30,6 +30,7 @@
 
 #include <cmath>
 #include <mutex>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -343,18 +344,143 @@ struct rows_block {
     static constexpr int LOGS = regs_of<LOGP>::ON ? (NT == 256 ? 11 : (NT == 512 ? 12 : 13)) : slab_of<LOGP>::LOGS;
 };
 
-// rows_fwd: block (matrix, group of 2T rows); matrices 0..nl-1 are the left ones (into SL),
-// the rest the right ones (into SR).  Matrix i starts at element moff[i]
-// of its input when the offset table is given (uneven batches), else at i * H * W.  With
-// mdims (uneven batches), matrix i is mdims[i].x x mdims[i].y, at most H x W (the padded
-// lengths are those of H x W); S keeps a column stride of H.
+// The block's T transformed row pairs (spectra in natural order in LDS) -> the half spectra
+// of rows r0 .. r0 + 2T - 1 (those below Hm), written column-major into Sm (column stride H):
+// with z = FFT(a + i b), A[k] = (z[k] + conj(z[P-k]))/2 and B[k] = (z[k] - conj(z[P-k]))/(2i).
+template <int LOGPC, int NT>
+__device__ __forceinline__ void split_rows(const double2* buf, double2* __restrict__ Sm, uint32_t r0, uint32_t Hm,
+                                           uint32_t H, int logT) {
+    constexpr int Pc = 1 << LOGPC;
+    constexpr uint32_t Kc = Pc / 2 + 1;
+    const int rows2 = 2 << logT;
+    for (int idx = threadIdx.x; idx < (int)Kc * rows2; idx += NT) {
+        const int k = idx >> (logT + 1), rr = idx & (rows2 - 1);
+        const uint32_t r = r0 + rr;
+        if (r >= Hm) continue;
+        const int t = rr >> 1;
+        const double2 zk = buf[at<LOGPC>(t, k)];
+        const double2 zn = cconj(buf[at<LOGPC>(t, (Pc - k) & (Pc - 1))]);
+        double2 v;
+        if ((rr & 1) == 0) {
+            v = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y + zn.y));
+        } else {
+            const double2 d = mul_mi(csub(zk, zn));
+            v = make_double2(0.5 * d.x, 0.5 * d.y);
+        }
+        Sm[(size_t)k * H + r] = v;
+    }
+}
+
+// rows_fwd for register transforms (up to 2048 points): a persistent grid of the blocks that
+// fit at once, each taking row groups (items: matrix, group of 2T rows) in turn and loading
+// the next item's rows into registers while it transforms and stores the current one, so
+// that HBM latency is paid once per block rather than once per item.  Items are dealt so
+// that each XCD (blocks b with b mod 8 = x) walks one contiguous eighth of the item range:
+// the partial lines of S that neighbouring groups write meet in its L2.  Matrices, offsets
+// and sizes as in rows_fwd_slab_kernel below.
 template <typename Tin, int LOGPC, int NT = rows_block<LOGPC, false>::NT>
-__global__ __launch_bounds__(NT) void rows_fwd_kernel(const Tin* __restrict__ XL, const Tin* __restrict__ XR,
-                                                       const uint64_t* __restrict__ moff,
-                                                       const uint2* __restrict__ mdims, uint32_t nl, uint32_t H,
-                                                       uint32_t W, int logT, uint32_t groups,
-                                                       double2* __restrict__ SL, double2* __restrict__ SR,
-                                                       const double2* __restrict__ tw) {
+__global__ __launch_bounds__(NT) void rows_fwd_regs_kernel(const Tin* __restrict__ XL, const Tin* __restrict__ XR,
+                                                           const uint64_t* __restrict__ moff,
+                                                           const uint2* __restrict__ mdims, uint32_t nl, uint32_t H,
+                                                           uint32_t W, int logT, uint32_t groups, uint32_t items,
+                                                           double2* __restrict__ SL, double2* __restrict__ SR,
+                                                           const double2* __restrict__ tw) {
+    static_assert(regs_of<LOGPC>::ON, "register transforms only");
+    constexpr int Q = 1 << (LOGPC - 3);
+    extern __shared__ double2 buf[];
+    const regs_pos<LOGPC> ps;
+    const int t0 = ps.t0, t = ps.t, i = ps.i;
+    const regs_roots<LOGPC> rr = load_roots<LOGPC>(i, tw);
+    // this block's items: x * per + k + j * bpx, j = 0, 1, ... below min((x + 1) * per, items)
+    const uint32_t nx = gridDim.x >= 8 ? 8 : 1;
+    const uint32_t x = blockIdx.x % nx, kb = blockIdx.x / nx, bpx = gridDim.x / nx;
+    const uint32_t per = (items + nx - 1) / nx;
+    const uint32_t end = min((x + 1) * per, items);
+    // where item `it` lives: input rows a, b of this thread, sizes, output
+    struct where {
+        const Tin* xa;
+        const Tin* xb;
+        uint32_t Hm, Wm, r0, mat;
+        bool left, oka, okb;
+    };
+    auto locate = [&](uint32_t it) {
+        where w;
+        const uint32_t matg = it / groups, g = it - matg * groups;
+        w.left = matg < nl;
+        w.mat = w.left ? matg : matg - nl;
+        w.r0 = g << (logT + 1);
+        w.Hm = H;
+        w.Wm = W;
+        if (mdims) {
+            const uint2 d = mdims[matg];
+            w.Hm = d.x;
+            w.Wm = d.y;
+        }
+        const Tin* Xm = (w.left ? XL : XR) + (moff ? moff[matg] : (size_t)w.mat * H * W);
+        const uint32_t a = w.r0 + 2 * t, b = a + 1;
+        w.oka = a < w.Hm;
+        w.okb = b < w.Hm;
+        w.xa = Xm + (w.oka ? (size_t)a * w.Wm : 0);
+        w.xb = Xm + (w.okb ? (size_t)b * w.Wm : 0);
+        return w;
+    };
+    // Branch-free loads from clamped (in-bounds) addresses; out-of-range positions are zeroed
+    // when the values are used.
+    auto fetch = [&](const where& w, Tin (&va)[8], Tin (&vb)[8]) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint32_t c = i + Q * r, cc = c < w.Wm ? c : w.Wm - 1;
+            va[r] = w.xa[cc];
+            vb[r] = w.xb[cc];
+        }
+    };
+    uint32_t it = x * per + kb;
+    if (it >= end) return;
+    where cur = locate(it);
+    Tin va[8], vb[8];
+    fetch(cur, va, vb);
+    while (true) {
+        const uint32_t nxt = it + bpx;
+        const bool more = nxt < end;
+        double2 u[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const bool in = i + Q * r < (int)cur.Wm;
+            u[r] = make_double2(cur.oka && in ? (double)va[r] : 0.0, cur.okb && in ? (double)vb[r] : 0.0);
+        }
+        where nw{};
+        if (more) {  // the next item's rows are in flight during this item's transform and stores
+            nw = locate(nxt);
+            fetch(nw, va, vb);
+        }
+        if (cur.r0 < cur.Hm) {  // block-uniform: a row group beyond a smaller matrix is skipped
+            fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), rr, t - t0, i);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) buf[at<LOGPC>(t, i + Q * r)] = u[r];
+            __syncthreads();
+            split_rows<LOGPC, NT>(buf, (cur.left ? SL : SR) + (size_t)cur.mat * ((1 << LOGPC) / 2 + 1) * H, cur.r0,
+                                  cur.Hm, H, logT);
+            __syncthreads();  // the next item's transform reuses the slab
+        }
+        if (!more) break;
+        it = nxt;
+        cur = nw;
+    }
+}
+
+// rows_fwd for 4096-point transforms (the LDS slab): block (matrix, group of 2T rows);
+// matrices 0..nl-1 are the left ones (into SL), the rest the right ones (into SR).  Matrix i
+// starts at element moff[i] of its input when the offset table is given (uneven batches),
+// else at i * H * W.  With mdims (uneven batches), matrix i is mdims[i].x x mdims[i].y, at
+// most H x W (the padded lengths are those of H x W); S keeps a column stride of H.
+template <typename Tin, int LOGPC, int NT = rows_block<LOGPC, false>::NT>
+__global__ __launch_bounds__(NT) void rows_fwd_slab_kernel(const Tin* __restrict__ XL, const Tin* __restrict__ XR,
+                                                           const uint64_t* __restrict__ moff,
+                                                           const uint2* __restrict__ mdims, uint32_t nl, uint32_t H,
+                                                           uint32_t W, int logT, uint32_t groups,
+                                                           double2* __restrict__ SL, double2* __restrict__ SR,
+                                                           const double2* __restrict__ tw) {
+    static_assert(!regs_of<LOGPC>::ON && NT == kFT, "slab transforms");
     constexpr int logPc = LOGPC;
     extern __shared__ double2 buf[];
     // XCD-aware order: the row groups of one matrix run on one XCD, so the partial lines of S
@@ -374,84 +500,33 @@ __global__ __launch_bounds__(NT) void rows_fwd_kernel(const Tin* __restrict__ XL
     if (r0 >= Hm) return;  // the whole block: a row group beyond a smaller matrix of the job
     const Tin* Xm = (left ? XL : XR) + (moff ? moff[matg] : (size_t)mat * H * W);
     double2* S = left ? SL : SR;
-    if constexpr (regs_of<LOGPC>::ON) {
-        // T = 8 NT / Pc: the transforms (row pairs) go through fft_regs from HBM; the spectra
-        // are left in natural order in LDS for the block's split below.
-        constexpr int Q = 1 << (LOGPC - 3);
-        const regs_pos<LOGPC> ps;
-        const int t0 = ps.t0, t = ps.t, i = ps.i;
+    // T = 2^LOGS / Pc transforms at most; element idx of the slab is column c of transform t.
+    // Branch-free clamped loads first, then the LDS writes.
+    constexpr int NI = (1 << slab_of<LOGPC>::LOGS) / kFT;
+    const slab_roots<LOGPC> sr = load_slab_roots<LOGPC>(tw);
+    const int n_el = T << logPc;
+    Tin va[NI], vb[NI];
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int idx = threadIdx.x + kFT * q;
+        const uint32_t t = idx >> logPc, c = idx & (Pc - 1);
         const uint32_t a = r0 + 2 * t, b = a + 1;
-        const regs_roots<LOGPC> rr = load_roots<LOGPC>(i, tw);
-        // Branch-free loads from clamped (in-bounds) addresses, all issued before the first
-        // value is used; out-of-range positions are zeroed afterwards.  (Guarded loads make
-        // the compiler wait on memory inside each guard.)
-        const bool oka = a < Hm, okb = b < Hm;
-        const Tin* xa = Xm + (oka ? (size_t)a * Wm : 0);
-        const Tin* xb = Xm + (okb ? (size_t)b * Wm : 0);
-        Tin va[8], vb[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const uint32_t c = i + Q * r, cc = c < Wm ? c : Wm - 1;
-            va[r] = xa[cc];
-            vb[r] = xb[cc];
-        }
-        double2 u[8];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            const bool in = i + Q * r < (int)Wm;
-            u[r] = make_double2(oka && in ? (double)va[r] : 0.0, okb && in ? (double)vb[r] : 0.0);
-        }
-        fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), rr, t - t0, i);
-#pragma unroll
-        for (int r = 0; r < 8; ++r) buf[at<LOGPC>(t, i + Q * r)] = u[r];
-        __syncthreads();
-    } else {
-        // Slab: T = 2^LOGS / Pc transforms at most; element idx of the slab is column c of
-        // transform t.  Branch-free clamped loads first, then the LDS writes.
-        constexpr int NI = (1 << slab_of<LOGPC>::LOGS) / kFT;
-        const slab_roots<LOGPC> sr = load_slab_roots<LOGPC>(tw);
-        const int n_el = T << logPc;
-        Tin va[NI], vb[NI];
-#pragma unroll
-        for (int q = 0; q < NI; ++q) {
-            const int idx = threadIdx.x + kFT * q;
-            const uint32_t t = idx >> logPc, c = idx & (Pc - 1);
-            const uint32_t a = r0 + 2 * t, b = a + 1;
-            const size_t cc = c < Wm ? c : Wm - 1;
-            va[q] = Xm[(size_t)(a < Hm ? a : Hm - 1) * Wm + cc];
-            vb[q] = Xm[(size_t)(b < Hm ? b : Hm - 1) * Wm + cc];
-        }
-#pragma unroll
-        for (int q = 0; q < NI; ++q) {
-            const int idx = threadIdx.x + kFT * q;
-            if (idx >= n_el) continue;
-            const uint32_t t = idx >> logPc, c = idx & (Pc - 1);
-            const uint32_t a = r0 + 2 * t, b = a + 1;
-            const bool in = c < Wm;
-            buf[at<LOGPC>(t, c)] = make_double2(a < Hm && in ? (double)va[q] : 0.0, b < Hm && in ? (double)vb[q] : 0.0);
-        }
-        __syncthreads();
-        fft_slab<LOGPC>(buf, T, sr);
+        const size_t cc = c < Wm ? c : Wm - 1;
+        va[q] = Xm[(size_t)(a < Hm ? a : Hm - 1) * Wm + cc];
+        vb[q] = Xm[(size_t)(b < Hm ? b : Hm - 1) * Wm + cc];
     }
-    const uint32_t Kc = (uint32_t)(Pc / 2 + 1);
-    const int rows2 = 2 << logT;
-    double2* Sm = S + (size_t)mat * Kc * H;
-    for (int idx = threadIdx.x; idx < (int)Kc * rows2; idx += NT) {
-        const int k = idx >> (logT + 1), rr = idx & (rows2 - 1);
-        const uint32_t r = r0 + rr;
-        if (r >= Hm) continue;
-        const int t = rr >> 1;
-        const double2 zk = buf[at<LOGPC>(t, k)];
-        const double2 zn = cconj(buf[at<LOGPC>(t, (Pc - k) & (Pc - 1))]);
-        double2 v;
-        if ((rr & 1) == 0) {
-            v = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y + zn.y));
-        } else {
-            const double2 d = mul_mi(csub(zk, zn));
-            v = make_double2(0.5 * d.x, 0.5 * d.y);
-        }
-        Sm[(size_t)k * H + r] = v;
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+        const int idx = threadIdx.x + kFT * q;
+        if (idx >= n_el) continue;
+        const uint32_t t = idx >> logPc, c = idx & (Pc - 1);
+        const uint32_t a = r0 + 2 * t, b = a + 1;
+        const bool in = c < Wm;
+        buf[at<LOGPC>(t, c)] = make_double2(a < Hm && in ? (double)va[q] : 0.0, b < Hm && in ? (double)vb[q] : 0.0);
     }
+    __syncthreads();
+    fft_slab<LOGPC>(buf, T, sr);
+    split_rows<LOGPC, NT>(buf, S + (size_t)mat * (Pc / 2 + 1) * H, r0, Hm, H, logT);
 }
 
 // cols: block (item, group of C columns k).  mode 0: item = left matrix l, writes its full
@@ -826,6 +901,33 @@ fftk_geom geom_of(uint32_t H, uint32_t W) {
     return g;
 }
 
+// Blocks of `kernel` (threads, dynamic LDS bytes) that the device holds at once, for a
+// persistent grid over `items` work items: at most `items`, a multiple of the 8 XCDs when
+// there are at least 8.  The occupancy is queried once per kernel and size.
+uint32_t resident_grid(const void* kernel, int threads, uint32_t lds, uint32_t items) {
+    static std::mutex mu;
+    static std::vector<std::tuple<const void*, int, uint32_t, int, int>> known;  // kernel, threads, lds, dev, blocks
+    int dev = 0;
+    XC_HIP(hipGetDevice(&dev));
+    int per_cu = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        for (const auto& e : known)
+            if (std::get<0>(e) == kernel && std::get<1>(e) == threads && std::get<2>(e) == lds && std::get<3>(e) == dev)
+                per_cu = std::get<4>(e);
+        if (per_cu == 0) {
+            XC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
+            per_cu = std::max(per_cu, 1);
+            known.emplace_back(kernel, threads, lds, dev, per_cu);
+        }
+    }
+    int cus = 0;
+    XC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    uint32_t g = std::min<uint32_t>(items, (uint32_t)cus * (uint32_t)per_cu);
+    if (g >= 8) g -= g % 8;
+    return std::max<uint32_t>(g, 1);
+}
+
 // Launch, with the kernel's own start and end recorded under `name` while kernel timing is on.
 template <typename K, typename... Args>
 void launch_timed(const char* name, bool timing, K kernel, dim3 grid, dim3 block, uint32_t lds, hipStream_t s,
@@ -853,10 +955,21 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
     const uint2* ptab = reinterpret_cast<const uint2*>(j.ptab);
     const uint2* mdims = reinterpret_cast<const uint2*>(j.mdims);
     with_logp(g.logPc, [&](auto LP) {
-        auto k = rows_fwd_kernel<Tio, decltype(LP)::value>;
-        allow_lds(k, g.v1.lds);
-        launch_timed("fftk_rows_fwd", timing, k, dim3((j.nl + j.nr) * g.g1), dim3(rows_threads(g.logPc, false)),
-                     g.v1.lds, s, L, R, j.moff, mdims, j.nl, H, W, g.v1.logT, g.g1, SL, SR, tw);
+        constexpr int LOGPC = decltype(LP)::value;
+        const uint32_t items = (j.nl + j.nr) * g.g1;
+        if constexpr (regs_of<LOGPC>::ON) {
+            auto k = rows_fwd_regs_kernel<Tio, LOGPC>;
+            allow_lds(k, g.v1.lds);
+            const uint32_t grid = resident_grid(reinterpret_cast<const void*>(k), rows_threads(LOGPC, false),
+                                                g.v1.lds, items);
+            launch_timed("fftk_rows_fwd", timing, k, dim3(grid), dim3(rows_threads(LOGPC, false)), g.v1.lds, s, L, R,
+                         j.moff, mdims, j.nl, H, W, g.v1.logT, g.g1, items, SL, SR, tw);
+        } else {
+            auto k = rows_fwd_slab_kernel<Tio, LOGPC>;
+            allow_lds(k, g.v1.lds);
+            launch_timed("fftk_rows_fwd", timing, k, dim3(items), dim3(kFT), g.v1.lds, s, L, R, j.moff, mdims, j.nl, H,
+                         W, g.v1.logT, g.g1, SL, SR, tw);
+        }
     });
     auto cols = [&](const char* name, const double2* S, double2* Zout, int mode, uint32_t items, uint32_t q0) {
         with_logp(g.logPr, [&](auto LP) {
