@@ -70,6 +70,8 @@ CASES = [
     ("one_to_many", 129, 256, 1, 3, "u11"),   # 512-point register path: rows and columns
     ("n_to_m", 200, 140, 2, 2, "normal"),     # 512-point columns, 512-point rows, partial H
     ("one_to_one", 256, 70, 1, 1, "u01"),     # 512-point columns only
+    ("one_to_many", 140, 270, 1, 2, "u01"),   # 1024-point rows (8-row blocks), last row group partial
+    ("n_to_m", 45, 700, 2, 2, "u11"),         # 2048-point rows, 45 rows: 6 groups of 8, the last partial
 ]
 
 
