@@ -329,6 +329,11 @@ def autotune(family, left, right, out, reps=3, stream=None):
     return [ms[i] for i in range(nc)], best.value
 
 
+def autotune_clear():
+    """Forget the process's autotune table (AUTO goes back to the cost model)."""
+    check(lib().xc_autotune_clear())
+
+
 def time_candidate(family, left, right, out, index, reps=3, stream=None):
     """ms of one candidate (xc_time_candidate_f32) on torch device tensors."""
     fam = FAMILIES[family] if isinstance(family, str) else family

@@ -390,8 +390,20 @@ uint32_t autotune_f32(const family_args& a, const float* L, const float* R, floa
     const uint32_t n = num_candidates();
     uint32_t best = 0;
     float best_ms = INFINITY;
-    for (uint32_t i = 0; i < n; ++i) {
-        const float t = time_candidate_f32(a, L, R, O, i, reps, s);
+    // The FFT candidates first; beyond 1e13 MACs the definition-based ones (seconds to
+    // minutes per run, 10-100x slower than the FFT there) are only timed when no FFT
+    // candidate could run.
+    const double macs = (double)a.pairs * a.rows * a.rows * a.cols * a.cols;
+    std::vector<uint32_t> order;
+    for (uint32_t i = 0; i < n; ++i)
+        if (candidate(i).precision == XC_PREC_FFT64) order.push_back(i);
+    const size_t n_fft = order.size();
+    for (uint32_t i = 0; i < n; ++i)
+        if (candidate(i).precision != XC_PREC_FFT64) order.push_back(i);
+    for (size_t oi = 0; oi < order.size(); ++oi) {
+        const uint32_t i = order[oi];
+        float t = INFINITY;
+        if (oi < n_fft || macs <= 1e13 || !std::isfinite(best_ms)) t = time_candidate_f32(a, L, R, O, i, reps, s);
         if (ms) ms[i] = t;
         if (t < best_ms) {
             best_ms = t;

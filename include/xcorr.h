@@ -238,8 +238,9 @@ xc_status xc_candidate(uint32_t i, xc_options* opt, char* name, size_t name_len)
 xc_status xc_workload_features(const xc_workload* w, float* features, uint32_t capacity, uint32_t* count);
 /* Exhaustive autotune: runs every candidate on the given buffers (one untimed warm-up, then
  * `reps` runs timed with HIP events on `stream`), writes the mean ms of each candidate
- * (INFINITY for one that failed) to ms[xc_num_candidates()], the fastest to *best, and
- * records the winner in the autotune table.  `out` holds the last candidate's results. */
+ * (INFINITY for one that failed or was not run) to ms[xc_num_candidates()], the fastest to
+ * *best, and records the winner in the autotune table.  Above 1e13 MACs the definition-based
+ * candidates are run only if no FFT candidate could run.  `out` holds the last run's results. */
 xc_status xc_autotune_f32(xc_family family, const float* left, const float* right, float* out, uint32_t rows,
                           uint32_t cols, uint32_t n, uint32_t m, uint32_t reps, float* ms, uint32_t* best,
                           xc_stream_t stream);
