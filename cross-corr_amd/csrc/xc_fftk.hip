@@ -256,8 +256,17 @@ template <int LOGP, int LOGPP>
 __device__ __forceinline__ void fft_regs_passes(double2 (&u)[8], double2* wb, int tl, int i,
                                                 const regs_roots<LOGP>& rr) {
     constexpr int Q = 1 << (LOGP - 3);
+    if constexpr (Q >= 8) {
+        // at(tl, i + Q r) = at(tl, i) + r (Q + Q/8) when 8 divides Q: one address register and
+        // constant offsets (the compiler does not split the shift of the sum by itself, and
+        // keeps eight addresses live through both transforms otherwise).
+        const int a = at<LOGP>(tl, i);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) u[r] = wb[at<LOGP>(tl, i + Q * r)];
+        for (int r = 0; r < 8; ++r) u[r] = wb[a + r * (Q + (Q >> 3))];
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) u[r] = wb[at<LOGP>(tl, i + Q * r)];
+    }
     const int k = i & ((1 << LOGPP) - 1);
     // The root passes through an empty asm so that its powers are formed here, in this pass:
     // otherwise the compiler hoists the power chains of every pass (and keeps them for a second
@@ -538,11 +547,10 @@ __global__ __launch_bounds__(NT) void rows_fwd_slab_kernel(const Tin* __restrict
 // second FFT of conj(Lhat) * X, conjugated: Z[item][k][rho] = Pr * IFFT_rho(Lhat * conj(X)).
 // Blocks are ordered column-group major through the XCD remap, so the pairs that read the
 // same Lhat columns run on one XCD and find them in its L2.
-template <int LOGPR>
+template <int LOGPR, int MODE>
 __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S, double2* __restrict__ Lhat,
                                                    double2* __restrict__ Z, const uint2* __restrict__ ptab,
-                                                   const uint2* __restrict__ mdims, uint32_t nl, int mode,
-                                                   int family, uint32_t n, uint32_t m, uint32_t q0, uint32_t items,
+                                                   const uint2* __restrict__ mdims, uint32_t nl, int family, uint32_t n, uint32_t m, uint32_t q0, uint32_t items,
                                                    uint32_t H, uint32_t Kc, int logC, const double2* __restrict__ tw) {
     constexpr int logPr = LOGPR;
     extern __shared__ double2 buf[];
@@ -551,7 +559,7 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
     const int Pr = 1 << logPr, C = 1 << logC;
     const uint32_t k0 = cg << logC;
     uint32_t l = item, r = item;
-    if (mode == 1) {
+    if constexpr (MODE == 1) {
         if (ptab) {
             l = ptab[q0 + item].x;
             r = ptab[q0 + item].y;
@@ -559,7 +567,7 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
             pair_of(family, n, m, q0 + item, l, r);
         }
     }
-    const uint32_t Hq = mdims ? mdims[mode == 1 ? nl + r : l].x : H;  // rows of this item's matrices
+    const uint32_t Hq = mdims ? mdims[MODE == 1 ? nl + r : l].x : H;  // rows of this item's matrices
     const double2* Sm = S + (size_t)r * Kc * H;
     double2* Ll = Lhat + (size_t)l * Kc * Pr;
     if constexpr (regs_of<LOGPR>::ON) {
@@ -585,10 +593,10 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             lh[r] = make_double2(0.0, 0.0);
-            if (mode == 1 && ok) lh[r] = lk[i + Q * r];
+            if (MODE == 1 && ok) lh[r] = lk[i + Q * r];
         }
         fft_regs<LOGPR>(u, wb, rr, ps.t - t0, i);
-        if (mode == 0) {
+        if constexpr (MODE == 0) {
             if (ok) {
 #pragma unroll
                 for (int r = 0; r < 8; ++r) lk[i + Q * r] = u[r];
@@ -603,9 +611,11 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
         asm volatile("" : "+v"(tl2), "+v"(i2));
         fft_regs<LOGPR>(u, wb, rr, tl2, i2);
         if (ok) {
-            double2* zk = Z + (size_t)item * Kc * Pr + (size_t)k * Pr;
+            // addressed from the opaque i2 with constant offsets, so that the row indices of
+            // the loads are not kept live through both transforms for these stores
+            double2* zk = Z + (size_t)item * Kc * Pr + (size_t)k * Pr + i2;
 #pragma unroll
-            for (int r = 0; r < 8; ++r) zk[i + Q * r] = cconj(u[r]);
+            for (int r = 0; r < 8; ++r) zk[Q * r] = cconj(u[r]);
         }
         return;
     } else {
@@ -621,7 +631,7 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
         const uint32_t k = k0 + (idx >> logPr), rho = idx & (Pr - 1);
         const uint32_t kk = k < Kc ? k : Kc - 1;
         xs[q] = Sm[(size_t)kk * H + (rho < Hq ? rho : Hq - 1)];
-        lh[q] = mode == 1 ? Ll[(size_t)kk * Pr + rho] : make_double2(0.0, 0.0);
+        lh[q] = MODE == 1 ? Ll[(size_t)kk * Pr + rho] : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
@@ -631,7 +641,7 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
     }
     __syncthreads();
     fft_slab<LOGPR>(buf, C, sr);
-    if (mode == 0) {
+    if constexpr (MODE == 0) {
 #pragma unroll
         for (int q = 0; q < NI; ++q) {
             const int idx = threadIdx.x + kFT * q;
@@ -971,19 +981,19 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
                          W, g.v1.logT, g.g1, SL, SR, tw);
         }
     });
-    auto cols = [&](const char* name, const double2* S, double2* Zout, int mode, uint32_t items, uint32_t q0) {
+    auto cols = [&](const char* name, const double2* S, double2* Zout, auto MODE, uint32_t items, uint32_t q0) {
         with_logp(g.logPr, [&](auto LP) {
-            auto k = cols_kernel<decltype(LP)::value>;
+            auto k = cols_kernel<decltype(LP)::value, decltype(MODE)::value>;
             allow_lds(k, g.vc.lds);
             launch_timed(name, timing, k, dim3(items * g.cg), dim3(kFT), g.vc.lds, s, S, LH, Zout, ptab,
-                         mdims, j.nl, mode, j.family, j.n, j.m, q0, items, H, g.Kc, g.vc.logT, tw);
+                         mdims, j.nl, j.family, j.n, j.m, q0, items, H, g.Kc, g.vc.logT, tw);
         });
     };
-    cols("fftk_cols_left", SL, nullptr, 0, j.nl, 0);
+    cols("fftk_cols_left", SL, nullptr, std::integral_constant<int, 0>{}, j.nl, 0);
     const double scale = 1.0 / ((double)g.Pr * (double)g.Pc);
     for (uint32_t q0 = 0; q0 < j.pairs; q0 += chunk) {
         const uint32_t nq = std::min(chunk, j.pairs - q0);
-        cols("fftk_cols", SR, Zw, 1, nq, q0);
+        cols("fftk_cols", SR, Zw, std::integral_constant<int, 1>{}, nq, q0);
         with_logp(g.logPc, [&](auto LP) {
             auto k = rows_inv_kernel<Tio, decltype(LP)::value>;
             allow_lds(k, g.v3.lds);
