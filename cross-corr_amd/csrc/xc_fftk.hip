@@ -550,8 +550,9 @@ __global__ __launch_bounds__(NT) void rows_fwd_slab_kernel(const Tin* __restrict
 template <int LOGPR, int MODE>
 __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S, double2* __restrict__ Lhat,
                                                    double2* __restrict__ Z, const uint2* __restrict__ ptab,
-                                                   const uint2* __restrict__ mdims, uint32_t nl, int family, uint32_t n, uint32_t m, uint32_t q0, uint32_t items,
-                                                   uint32_t H, uint32_t Kc, int logC, const double2* __restrict__ tw) {
+                                                   const uint2* __restrict__ mdims, uint32_t nl, int family,
+                                                   uint32_t n, uint32_t m, uint32_t q0, uint32_t items, uint32_t H,
+                                                   uint32_t Kc, int logC, const double2* __restrict__ tw) {
     constexpr int logPr = LOGPR;
     extern __shared__ double2 buf[];
     const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -981,7 +982,8 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
                          W, g.v1.logT, g.g1, SL, SR, tw);
         }
     });
-    auto cols = [&](const char* name, const double2* S, double2* Zout, auto MODE, uint32_t items, uint32_t q0) {
+    auto cols = [&](const char* name, const double2* S, double2* Zout, auto MODE, uint32_t items,
+                    uint32_t q0) {
         with_logp(g.logPr, [&](auto LP) {
             auto k = cols_kernel<decltype(LP)::value, decltype(MODE)::value>;
             allow_lds(k, g.vc.lds);
