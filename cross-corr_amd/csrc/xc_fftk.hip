@@ -538,6 +538,11 @@ __global__ __launch_bounds__(NT) void rows_fwd_slab_kernel(const Tin* __restrict
     split_rows<LOGPC, NT>(buf, S + (size_t)mat * (Pc / 2 + 1) * H, r0, Hm, H, logT);
 }
 
+// Threads per block of the column kernel: one wave for wave-owned register transforms (up to
+// 512 points; C = 512 / Pr columns per block, so that a pair's last column group wastes at
+// most a wave), 256 for the block-owned and slab transforms.
+constexpr int cols_threads(int logP) { return logP <= 9 ? 64 : kFT; }
+
 // cols: block (item, group of C columns k).  mode 0: item = left matrix l, writes its full
 // column spectra Lhat[l][k][rho] (the FFT along rho of SL[l][k][0..H), zero padded).  With
 // mdims (uneven batches) a matrix has mdims[i].x rows of its own (i = l, or nl + r for the
@@ -548,7 +553,7 @@ __global__ __launch_bounds__(NT) void rows_fwd_slab_kernel(const Tin* __restrict
 // Blocks are ordered column-group major through the XCD remap, so the pairs that read the
 // same Lhat columns run on one XCD and find them in its L2.
 template <int LOGPR, int MODE>
-__global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S, double2* __restrict__ Lhat,
+__global__ __launch_bounds__(cols_threads(LOGPR)) void cols_kernel(const double2* __restrict__ S, double2* __restrict__ Lhat,
                                                    double2* __restrict__ Z, const uint2* __restrict__ ptab,
                                                    const uint2* __restrict__ mdims, uint32_t nl, int family,
                                                    uint32_t n, uint32_t m, uint32_t q0, uint32_t items, uint32_t H,
@@ -572,7 +577,7 @@ __global__ __launch_bounds__(kFT) void cols_kernel(const double2* __restrict__ S
     const double2* Sm = S + (size_t)r * Kc * H;
     double2* Ll = Lhat + (size_t)l * Kc * Pr;
     if constexpr (regs_of<LOGPR>::ON) {
-        // C = 2048/Pr: the columns go from HBM to HBM through fft_regs; the product with
+        // C = 8 * threads / Pr: the columns go from HBM to HBM through fft_regs; the product with
         // conj(Lhat) happens in registers between the two FFTs.
         constexpr int Q = 1 << (LOGPR - 3);
         const regs_pos<LOGPR> ps;
@@ -902,7 +907,7 @@ fftk_geom geom_of(uint32_t H, uint32_t W) {
     // Slab shape per kernel (see shape_for): rows_fwd 2T input rows per block, cols C columns,
     // rows_inv 2T output rows.
     g.v1 = shape_for(g.logPc, (H + 1) / 2, rows_threads(g.logPc, false));
-    g.vc = shape_for(g.logPr, g.Kc);
+    g.vc = shape_for(g.logPr, g.Kc, cols_threads(g.logPr));
     g.v3 = shape_for(g.logPc, g.Pr / 2, rows_threads(g.logPc, true));
     g.g1 = (H + (2u << g.v1.logT) - 1) >> (g.v1.logT + 1);
     g.cg = (g.Kc + (1u << g.vc.logT) - 1) >> g.vc.logT;
@@ -987,8 +992,8 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
         with_logp(g.logPr, [&](auto LP) {
             auto k = cols_kernel<decltype(LP)::value, decltype(MODE)::value>;
             allow_lds(k, g.vc.lds);
-            launch_timed(name, timing, k, dim3(items * g.cg), dim3(kFT), g.vc.lds, s, S, LH, Zout, ptab,
-                         mdims, j.nl, j.family, j.n, j.m, q0, items, H, g.Kc, g.vc.logT, tw);
+            launch_timed(name, timing, k, dim3(items * g.cg), dim3(cols_threads(g.logPr)), g.vc.lds, s, S, LH, Zout,
+                         ptab, mdims, j.nl, j.family, j.n, j.m, q0, items, H, g.Kc, g.vc.logT, tw);
         });
     };
     cols("fftk_cols_left", SL, nullptr, std::integral_constant<int, 0>{}, j.nl, 0);
