@@ -70,7 +70,7 @@ def kernel_bytes(name, H, W, n, m, k):
     if name == "fftk_rows_fwd":
         return (n + m) * (H * W * 4 + Kc * H * 16)
     if name == "fftk_cols":
-        return m * Kc * H * 16 + n * Kc * Pr * 16 + k * Kc * Pr * 16
+        return m * Kc * H * 16 + n * Kc * Pr * 16 + k * Kc * SY * 16
     if name == "fftk_rows_inv":
         return k * Kc * SY * 16 + out
     pr, pc = 2 * H, 2 * W

@@ -12,8 +12,9 @@
 //              (k = 0..Pc/2, only the H nonzero rows r);
 //   cols     : per (pair, column k): X = FFT of length Pr of S[right][k][0..H-1] (zero
 //              padded), then a second FFT of conj(Lhat) * X, conjugated:
-//              Z[pair][k][rho] = Pr * IFFT(Lhat * conj(X))  (Lhat = the left's 2D spectrum,
-//              made by the same kernel in "left" mode);
+//              Pr * IFFT(Lhat * conj(X)) (Lhat = the left's 2D spectrum, made by the same
+//              kernel in "left" mode), of which only the 2H-1 circular rows of the result
+//              are kept: Z[pair][k][z_row(rho)];
 //   rows_inv : per pair, two output rows per complex FFT of length Pc: the Hermitian half
 //              rows of Z are extended, inverse transformed, scaled by 1/(Pr*Pc) and written
 //              straight into the cropped (2H-1) x (2W-1) result (quadrant swap by index).
@@ -538,6 +539,24 @@ __global__ __launch_bounds__(NT) void rows_fwd_slab_kernel(const Tin* __restrict
     split_rows<LOGPC, NT>(buf, S + (size_t)mat * (Pc / 2 + 1) * H, r0, Hm, H, logT);
 }
 
+// Row of Z that keeps circular row rho of a pair with H rows: the result's shifts dy = rho
+// (rho < H) stay in place, dy = rho - Pr (rho > Pr - H) move down by Pr - 2H to rows
+// H + 1 .. 2H - 1, so that both halves start on the same alignment as in the transform (and
+// for Pr = 2H nothing moves); -1 for the rows in between and for rho >= Pr.  Row H of Z is
+// never written.
+__device__ __forceinline__ int z_row(uint32_t rho, uint32_t H, uint32_t Pr) {
+    if (rho < H) return (int)rho;
+    if (rho > Pr - H && rho < Pr) return (int)(rho - (Pr - 2 * H));
+    return -1;
+}
+
+// Output row (dy + H - 1) of row z of Z (z_row's inverse): -1 for row H and rows >= 2H.
+__device__ __forceinline__ int out_row_of_z(uint32_t z, uint32_t H) {
+    if (z < H) return (int)(z + H - 1);
+    if (z > H && z < 2 * H) return (int)(z - H - 1);
+    return -1;
+}
+
 // Threads per block of the column kernel: one wave for wave-owned register transforms (up to
 // 512 points; C = 512 / Pr columns per block, so that a pair's last column group wastes at
 // most a wave), 256 for the block-owned and slab transforms.
@@ -548,8 +567,10 @@ constexpr int cols_threads(int logP) { return logP <= 9 ? 64 : kFT; }
 // mdims (uneven batches) a matrix has mdims[i].x rows of its own (i = l, or nl + r for the
 // right of a pair); S keeps a column stride of H.
 // mode 1: item = pair q0 + item (left l, right r from the family's order, or from the pair
-// table of an uneven batch): X = the same FFT of SR[r][k], then a
-// second FFT of conj(Lhat) * X, conjugated: Z[item][k][rho] = Pr * IFFT_rho(Lhat * conj(X)).
+// table of an uneven batch): X = the same FFT of SR[r][k], then a second FFT of
+// conj(Lhat) * X, conjugated: Pr * IFFT_rho(Lhat * conj(X)), whose circular row rho goes to
+// Z[item][k][z_row(rho)] (column stride Ys = 2H; the rows between the result's two halves
+// are dropped).
 // Blocks are ordered column-group major through the XCD remap, so the pairs that read the
 // same Lhat columns run on one XCD and find them in its L2.
 template <int LOGPR, int MODE>
@@ -557,7 +578,8 @@ __global__ __launch_bounds__(cols_threads(LOGPR)) void cols_kernel(const double2
                                                    double2* __restrict__ Z, const uint2* __restrict__ ptab,
                                                    const uint2* __restrict__ mdims, uint32_t nl, int family,
                                                    uint32_t n, uint32_t m, uint32_t q0, uint32_t items, uint32_t H,
-                                                   uint32_t Kc, int logC, const double2* __restrict__ tw) {
+                                                   uint32_t Kc, uint32_t Ys, int logC,
+                                                   const double2* __restrict__ tw) {
     constexpr int logPr = LOGPR;
     extern __shared__ double2 buf[];
     const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -617,11 +639,26 @@ __global__ __launch_bounds__(cols_threads(LOGPR)) void cols_kernel(const double2
         asm volatile("" : "+v"(tl2), "+v"(i2));
         fft_regs<LOGPR>(u, wb, rr, tl2, i2);
         if (ok) {
-            // addressed from the opaque i2 with constant offsets, so that the row indices of
-            // the loads are not kept live through both transforms for these stores
-            double2* zk = Z + (size_t)item * Kc * Pr + (size_t)k * Pr + i2;
+            // rows from the opaque i2, so that the row indices of the loads are not kept live
+            // through both transforms for these stores
+            // z_row with constant offsets from two bases: rows below Hq in place, rows from
+            // Pr - Hq on down by Pr - 2Hq (for Pr = 2Hq one base and every row, row Hq
+            // included, which the row kernel reads as zero)
+            double2* zlo = Z + (size_t)item * Kc * Ys + (size_t)k * Ys + i2;
+            if (Pr == 2 * (int)Hq) {
 #pragma unroll
-            for (int r = 0; r < 8; ++r) zk[Q * r] = cconj(u[r]);
+                for (int r = 0; r < 8; ++r) zlo[Q * r] = cconj(u[r]);
+            } else {
+                double2* zhi = zlo - (Pr - 2 * (int)Hq);
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const uint32_t rho = (uint32_t)(i2 + Q * r);
+                    if (rho < Hq)
+                        zlo[Q * r] = cconj(u[r]);
+                    else if (rho >= (uint32_t)Pr - Hq)
+                        zhi[Q * r] = cconj(u[r]);
+                }
+            }
         }
         return;
     } else {
@@ -665,36 +702,28 @@ __global__ __launch_bounds__(cols_threads(LOGPR)) void cols_kernel(const double2
     }
     __syncthreads();
     fft_slab<LOGPR>(buf, C, sr);
-    double2* Zq = Z + (size_t)item * Kc * Pr;
+    double2* Zq = Z + (size_t)item * Kc * Ys;
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
         const int idx = threadIdx.x + kFT * q;
         const uint32_t k = k0 + (idx >> logPr);
-        if (k < Kc) Zq[(size_t)k * Pr + (idx & (Pr - 1))] = cconj(buf[at<LOGPR>(idx >> logPr, idx & (Pr - 1))]);
+        const int y = z_row((uint32_t)(idx & (Pr - 1)), Hq, (uint32_t)Pr);
+        if (k < Kc && y >= 0) Zq[(size_t)k * Ys + y] = cconj(buf[at<LOGPR>(idx >> logPr, idx & (Pr - 1))]);
     }
     }
 }
 
-// Output row of circular row rho: dy = rho for rho < H, rho - Pr for rho > Pr - H; -1 for the
-// rows in between (not part of the (2H-1)-row result) and for rho >= Pr.
-__device__ __forceinline__ int out_row(uint32_t rho, uint32_t H, uint32_t Pr) {
-    if (rho >= Pr) return -1;
-    if (rho < H) return (int)(rho + H - 1);
-    if (rho > Pr - H) return (int)(rho - (Pr - H + 1));
-    return -1;
-}
-
-// rows_inv: block (pair item, group of 2T consecutive circular rows rho0 = 2T*g ...), so that
-// each thread's two rows are adjacent in Z and a group's rows are one aligned run.  With
-// mdims (uneven batches) the pair's size is that of its right matrix nl + ptab[q].y.  Row rho
-// goes to output row out_row(rho); output column ox = dx + W - 1 takes circular column
-// dx mod Pc.  Pair q's result starts at ooff[q] when the table is given, else at q times the
-// result size.
+// rows_inv: block (pair item, group of 2T consecutive rows z0 = 2T*g ... of Z, column stride
+// Ys), so that each thread's two rows are adjacent in Z and a group's rows are one aligned
+// run.  With mdims (uneven batches) the pair's size is that of its right matrix
+// nl + ptab[q].y.  Row z goes to output row out_row_of_z(z) (row H of Z, which holds nothing,
+// is read as zero); output column ox = dx + W - 1 takes circular column dx mod Pc.  Pair q's
+// result starts at ooff[q] when the table is given, else at q times the result size.
 template <typename Tout, int LOGPC, int NT = rows_block<LOGPC, true>::NT>
 __global__ __launch_bounds__(NT) void rows_inv_kernel(const double2* __restrict__ Z, Tout* __restrict__ O,
                                                        const uint64_t* __restrict__ ooff,
                                                        const uint2* __restrict__ ptab, const uint2* __restrict__ mdims,
-                                                       uint32_t nl, uint32_t q0, uint32_t H, uint32_t W, int logPr,
+                                                       uint32_t nl, uint32_t q0, uint32_t H, uint32_t W, uint32_t Ys,
                                                        int logT, uint32_t groups, double scale,
                                                        const double2* __restrict__ tw) {
     constexpr int logPc = LOGPC;
@@ -704,7 +733,6 @@ __global__ __launch_bounds__(NT) void rows_inv_kernel(const double2* __restrict_
     const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t item = bid / groups, g = bid - item * groups;
     const int Pc = 1 << logPc, T = 1 << logT;
-    const uint32_t Pr = 1u << logPr;
     const int half = Pc / 2;
     const uint32_t Kc = (uint32_t)half + 1;
     if (mdims) {  // uneven batch: this pair's own size (at most H x W)
@@ -713,11 +741,10 @@ __global__ __launch_bounds__(NT) void rows_inv_kernel(const double2* __restrict_
         W = d.y;
     }
     const uint32_t SX = 2 * W - 1;
-    const uint32_t rho0 = g << (logT + 1);
+    const uint32_t z0 = g << (logT + 1);
     const int rows2 = 2 << logT;
-    // Groups wholly between the two halves of the result have nothing to write.
-    if (out_row(rho0, H, Pr) < 0 && out_row(rho0 + rows2 - 1, H, Pr) < 0 && rho0 >= H) return;
-    const double2* Zq = Z + (size_t)item * Kc * Pr;
+    if (z0 >= 2 * H) return;  // a smaller pair of an uneven batch
+    const double2* Zq = Z + (size_t)item * Kc * Ys;
     regs_roots<LOGPC> rr{};
     slab_roots<LOGPC> sr{};
     if constexpr (regs_of<LOGPC>::ON)
@@ -738,12 +765,12 @@ __global__ __launch_bounds__(NT) void rows_inv_kernel(const double2* __restrict_
         for (int q = 0; q < NQ; ++q) {
             const int idx = threadIdx.x + NT * q;
             const int kk = idx >> logT, t = idx & (T - 1);
-            const uint32_t ra = rho0 + 2 * t;
-            const double2* zc = Zq + (size_t)kk * Pr + ra;
+            const uint32_t za = z0 + 2 * t;
+            const double2* zc = Zq + (size_t)kk * Ys + za;
             A[q] = make_double2(0.0, 0.0);
             B[q] = make_double2(0.0, 0.0);
-            if (idx < items && ra < Pr) A[q] = zc[0];
-            if (idx < items && ra + 1 < Pr) B[q] = zc[1];
+            if (idx < items && out_row_of_z(za, H) >= 0) A[q] = zc[0];
+            if (idx < items && out_row_of_z(za + 1, H) >= 0) B[q] = zc[1];
         }
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
@@ -768,7 +795,7 @@ __global__ __launch_bounds__(NT) void rows_inv_kernel(const double2* __restrict_
         // a block-owned transform's first pass overwrites slots other waves still read
         if constexpr (regs_of<LOGPC>::BLOCK) __syncthreads();
         fft_regs<LOGPC>(u, buf + at<LOGPC>(t0, 0), rr, t - t0, i);
-        const int oa = out_row(rho0 + 2 * t, H, Pr), ob = out_row(rho0 + 2 * t + 1, H, Pr);
+        const int oa = out_row_of_z(z0 + 2 * t, H), ob = out_row_of_z(z0 + 2 * t + 1, H);
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
             const int c = i + Q * r;
@@ -782,7 +809,7 @@ __global__ __launch_bounds__(NT) void rows_inv_kernel(const double2* __restrict_
     fft_slab<LOGPC>(buf, T, sr);
     // IFFT(Y) = conj(FFT(conj(Y))) / Pc: row A is the real part, row B minus the imaginary.
     for (int rr = threadIdx.x >> 6; rr < rows2; rr += NT / 64) {
-        const int oy = out_row(rho0 + rr, H, Pr);
+        const int oy = out_row_of_z(z0 + rr, H);
         if (oy < 0) continue;
         Tout* orow = Oq + (size_t)oy * SX;
         const int t = rr >> 1;
@@ -892,7 +919,8 @@ struct fftk_geom {
     uint32_t Pr, Pc, Kc;
     slab_shape v1, vc, v3;
     uint32_t g1, cg, g3;
-    size_t s_bytes, col_bytes;
+    uint32_t Ys;  // column stride of Z: 2H (the 2H-1 circular rows of the result, z_row)
+    size_t s_bytes, col_bytes, z_bytes;
 };
 
 fftk_geom geom_of(uint32_t H, uint32_t W) {
@@ -908,12 +936,14 @@ fftk_geom geom_of(uint32_t H, uint32_t W) {
     // rows_inv 2T output rows.
     g.v1 = shape_for(g.logPc, (H + 1) / 2, rows_threads(g.logPc, false));
     g.vc = shape_for(g.logPr, g.Kc, cols_threads(g.logPr));
-    g.v3 = shape_for(g.logPc, g.Pr / 2, rows_threads(g.logPc, true));
+    g.v3 = shape_for(g.logPc, H, rows_threads(g.logPc, true));
     g.g1 = (H + (2u << g.v1.logT) - 1) >> (g.v1.logT + 1);
     g.cg = (g.Kc + (1u << g.vc.logT) - 1) >> g.vc.logT;
-    g.g3 = (g.Pr + (2u << g.v3.logT) - 1) >> (g.v3.logT + 1);  // a block may be taller than Pr
+    g.g3 = (2 * H + (2u << g.v3.logT) - 1) >> (g.v3.logT + 1);  // a block may be taller
+    g.Ys = 2 * H;
     g.s_bytes = sizeof(double2) * (size_t)g.Kc * H;
     g.col_bytes = sizeof(double2) * (size_t)g.Kc * g.Pr;
+    g.z_bytes = sizeof(double2) * (size_t)g.Kc * g.Ys;
     return g;
 }
 
@@ -993,7 +1023,7 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
             auto k = cols_kernel<decltype(LP)::value, decltype(MODE)::value>;
             allow_lds(k, g.vc.lds);
             launch_timed(name, timing, k, dim3(items * g.cg), dim3(cols_threads(g.logPr)), g.vc.lds, s, S, LH, Zout,
-                         ptab, mdims, j.nl, j.family, j.n, j.m, q0, items, H, g.Kc, g.vc.logT, tw);
+                         ptab, mdims, j.nl, j.family, j.n, j.m, q0, items, H, g.Kc, g.Ys, g.vc.logT, tw);
         });
     };
     cols("fftk_cols_left", SL, nullptr, std::integral_constant<int, 0>{}, j.nl, 0);
@@ -1005,7 +1035,7 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
             auto k = rows_inv_kernel<Tio, decltype(LP)::value>;
             allow_lds(k, g.v3.lds);
             launch_timed("fftk_rows_inv", timing, k, dim3(nq * g.g3), dim3(rows_threads(g.logPc, true)), g.v3.lds, s,
-                         (const double2*)Zw, O, j.ooff, ptab, mdims, j.nl, q0, H, W, g.logPr,
+                         (const double2*)Zw, O, j.ooff, ptab, mdims, j.nl, q0, H, W, g.Ys,
                          g.v3.logT, g.g3, scale, tw);
         });
     }
@@ -1040,11 +1070,11 @@ fftk_ws_sizes fftk_sizes(const fftk_job& j) {
     const fftk_geom g = geom_of(j.H, j.W);
     fftk_ws_sizes z{};
     // Z is processed in chunks of pairs of at most ~2 GiB.
-    z.chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(j.pairs, (size_t(2) << 30) / g.col_bytes));
+    z.chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(j.pairs, (size_t(2) << 30) / g.z_bytes));
     z.sr = std::max<size_t>(g.s_bytes * j.nr, 16);
     z.sl = std::max<size_t>(g.s_bytes * j.nl, 16);
     z.lh = std::max<size_t>(g.col_bytes * j.nl, 16);
-    z.z = g.col_bytes * z.chunk;
+    z.z = g.z_bytes * z.chunk;
     return z;
 }
 
