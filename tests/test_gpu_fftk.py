@@ -127,6 +127,39 @@ def test_largest_transforms_against_rocfft_and_sum_rule(size):
     assert abs(own[1, 2 * n, 2 * n] - left[0, n, n] * right[1, 0, 0]) <= 1e-5
 
 
+def entry(left, right, dy, dx):
+    """One output of the definition, sum_{r,c} L[r][c] R[r-dy][c-dx], in FP64."""
+    H, W = left.shape
+    a = left[max(0, dy):H + min(0, dy), max(0, dx):W + min(0, dx)].astype(np.float64)
+    b = right[max(0, -dy):H - max(0, dy), max(0, -dx):W - max(0, dx)].astype(np.float64)
+    return float((a * b).sum())
+
+
+@pytest.mark.parametrize("rows,cols", [(1100, 600), (600, 1100)])
+def test_ragged_sizes_of_the_4096_point_transforms(rows, cols):
+    # 4096-point columns (rows = 1100) or rows (cols = 1100) with padding that is not 2x the
+    # size: the column pass keeps 2H-1 of the Pr circular rows.  Too large for the oracle:
+    # the sum rule, the rocFFT path, and outputs beside both halves of the result (the shifts
+    # dy = 0, +-1, +-(H-1)) against the definition.
+    left = seeded((1, rows, cols), "u01", 35, np.float32)
+    right = seeded((2, rows, cols), "u01", 36, np.float32)
+    own = run("one_to_many", left, right)
+    assert own.shape == (2, 2 * rows - 1, 2 * cols - 1)
+    sums = own.astype(np.float64).reshape(2, -1).sum(axis=1)
+    expect = left.astype(np.float64).sum() * right.astype(np.float64).reshape(2, -1).sum(axis=1)
+    assert np.max(np.abs(sums - expect) / expect) < 1e-6
+    roc = run("one_to_many", left, right, tile=0)
+    scale = float(rows) * cols / 4  # about the central value
+    assert np.max(np.abs(own.astype(np.float64) - roc)) / scale < 1e-6
+    H, W = rows, cols
+    for t in range(2):
+        for dy in (-(H - 1), -(H - 2), -1, 0, 1, H - 2, H - 1):
+            for dx in (-(W - 1), -1, 0, 1, W - 1):
+                want = entry(left[0], right[t], dy, dx)  # U[0,1) data: corr(|L|,|R|) = want
+                got = float(own[t, dy + H - 1, dx + W - 1])
+                assert abs(got - want) <= 1e-6 * want, (t, dy, dx, got, want)
+
+
 def test_linearity_in_the_right_matrix():
     left = seeded((1, 200, 150), "u11", 41, np.float32)
     right = seeded((2, 200, 150), "u11", 42, np.float32)
