@@ -1,7 +1,9 @@
 // Method dispatch and device workspace management.
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "xc_methods.hpp"
@@ -17,7 +19,10 @@ struct ws_slot {
 };
 
 std::mutex g_ws_mutex;
-std::vector<std::vector<ws_slot>> g_ws;  // [device][slot]
+// Workspaces per (device, stream): work on one stream runs in order, so one set per stream
+// is never used by two calls at once; calls on different streams (or host threads with
+// their own streams) get their own sets.  Kept until the process ends.
+std::map<std::pair<int, hipStream_t>, std::vector<ws_slot>> g_ws;
 
 }  // namespace
 
@@ -25,13 +30,13 @@ void* workspace(int slot, size_t bytes, hipStream_t s) {
     std::lock_guard<std::mutex> lock(g_ws_mutex);
     int dev = 0;
     XC_HIP(hipGetDevice(&dev));
-    if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
-    auto& v = g_ws[dev];
+    auto& v = g_ws[std::make_pair(dev, s)];
     if ((int)v.size() <= slot) v.resize(slot + 1);
     ws_slot& w = v[slot];
     if (w.bytes < bytes) {
         if (w.ptr) {
-            // The old buffer may still be in use by queued work on this stream.
+            // The old buffer may still be in use by queued work on this stream (and only
+            // this stream uses it).
             XC_HIP(hipStreamSynchronize(s));
             XC_HIP(hipFree(w.ptr));
             w.ptr = nullptr;

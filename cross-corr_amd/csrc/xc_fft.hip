@@ -99,10 +99,14 @@ void fft_check(hipfftResult r, const char* what) {
 
 std::mutex g_fft_mutex;
 
-// Cached plans keyed by (padded rows, padded cols, batch, type).
-hipfftHandle get_plan(uint32_t prow, uint32_t pcol, uint32_t batch, hipfftType type) {
-    static std::map<std::tuple<uint32_t, uint32_t, uint32_t, int>, hipfftHandle> plans;
-    auto key = std::make_tuple(prow, pcol, batch, (int)type);
+// Cached plans keyed by (device, stream, padded rows, padded cols, batch, type): a plan is
+// made on one device, and its internal work buffer must not be shared by work that can run
+// at the same time on two streams.
+hipfftHandle get_plan(hipStream_t s, uint32_t prow, uint32_t pcol, uint32_t batch, hipfftType type) {
+    static std::map<std::tuple<int, hipStream_t, uint32_t, uint32_t, uint32_t, int>, hipfftHandle> plans;
+    int dev = 0;
+    XC_HIP(hipGetDevice(&dev));
+    auto key = std::make_tuple(dev, s, prow, pcol, batch, (int)type);
     auto it = plans.find(key);
     if (it != plans.end()) return it->second;
     hipfftHandle h;
@@ -161,7 +165,7 @@ void run_fft(const family_args& a, const Ti* L, const Ti* R, Ti* O, bool normali
     }
     XC_HIP(hipGetLastError());
 
-    hipfftHandle fwd = get_plan(PR, PC, nl + nr, fft_types<Tt>::r2c);
+    hipfftHandle fwd = get_plan(s, PR, PC, nl + nr, fft_types<Tt>::r2c);
     fft_check(hipfftSetStream(fwd, s), "hipfftSetStream");
     {
         timed t1("rocfft_forward", s);
@@ -181,7 +185,7 @@ void run_fft(const family_args& a, const Ti* L, const Ti* R, Ti* O, bool normali
     XC_HIP(hipGetLastError());
     record_launch("fft_hadamard", dim3(grid_for(tot)), dim3(256), 0, H, W, k, 1);
 
-    hipfftHandle inv = get_plan(PR, PC, k, fft_types<Tt>::c2r);
+    hipfftHandle inv = get_plan(s, PR, PC, k, fft_types<Tt>::c2r);
     fft_check(hipfftSetStream(inv, s), "hipfftSetStream");
     const bool direct_out = !normalize && std::is_same<Ti, Tt>::value;
     Tt* res = direct_out ? (Tt*)O : (Tt*)workspace(WS_FFT_RES, (uint64_t)k * parea * sizeof(Tt), s);

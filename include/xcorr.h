@@ -15,6 +15,10 @@
  *    GPU can read); the caller owns them; results are fully overwritten, the
  *    caller never pre-zeroes anything;
  *  - work is enqueued on `stream` (NULL = the null stream) and is asynchronous;
+ *    the library's scratch memory (and its rocFFT plans) are kept per (device,
+ *    stream), so calls on different streams, or from threads using their own
+ *    streams, may run at the same time; a stream's scratch grows on demand and
+ *    is kept until the process ends;
  *  - no entry point throws: each returns an xc_status, and xc_last_error() gives
  *    the message of the calling thread's last failure;
  *  - arguments are validated on the host before any device work is enqueued.
@@ -187,7 +191,7 @@ xc_status xc_normalize_fft_f64(const double* res, double* norm, uint32_t rows, u
  * src/one_to_many.hpp:679-745, src/n_to_m.hpp:700-736).  If `normalize` is 0 the
  * results are the unnormalised (2*rows) x (2*cols) circular correlations the
  * reference stores without -n; otherwise (2*rows-1) x (2*cols-1) as every other
- * method.  Plans and workspace are cached per (shape, batch, precision). */
+ * method.  Plans are cached per (device, stream, shape, batch, precision). */
 xc_status xc_fft_cross_corr_f32(xc_family family, const float* left, const float* right, float* out,
                                 uint32_t rows, uint32_t cols, uint32_t n, uint32_t m, int normalize,
                                 int fp64_transform, xc_stream_t stream);

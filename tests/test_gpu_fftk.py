@@ -160,6 +160,29 @@ def test_ragged_sizes_of_the_4096_point_transforms(rows, cols):
                 assert abs(got - want) <= 1e-6 * want, (t, dy, dx, got, want)
 
 
+@pytest.mark.parametrize("tile", [1, 0])
+def test_calls_on_two_streams_at_once(tile):
+    # Scratch memory and rocFFT plans are kept per stream: two streams running the same
+    # shape at the same time must each get the single-stream results (the kernels are
+    # deterministic, so bit for bit).
+    x = xcorr()
+    ins = [(dev(seeded((1, 96, 80), "u01", 51 + 2 * i, np.float32)),
+            dev(seeded((6, 96, 80), "u01", 52 + 2 * i, np.float32))) for i in range(2)]
+    want = [x.cross_corr("one_to_many", L, R, precision="fft64", tile=tile) for L, R in ins]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    got = [[], []]
+    print(f"two streams, fft64 tile {tile}", flush=True)
+    for _ in range(4):
+        for i in range(2):
+            with torch.cuda.stream(streams[i]):
+                got[i].append(x.cross_corr("one_to_many", *ins[i], precision="fft64", tile=tile))
+    torch.cuda.synchronize()
+    for i in range(2):
+        for g in got[i]:
+            assert torch.equal(g, want[i])
+
+
 def test_linearity_in_the_right_matrix():
     left = seeded((1, 200, 150), "u11", 41, np.float32)
     right = seeded((2, 200, 150), "u11", 42, np.float32)
