@@ -236,11 +236,12 @@ def pair_array(pairs):
     return (Pair * len(pairs))(*[Pair(*p) for p in pairs])
 
 
-def batch(pairs, left_flat, right_flat, out_flat, precision="auto", stream=None):
+def batch(pairs, left_flat, right_flat, out_flat, precision="auto", tile=0, split=0, stream=None):
     """Uneven batch.  pairs: list of (left_off, right_off, out_off, rows, cols) in elements,
-    or a pair_array."""
+    or a pair_array.  precision "fft64" with tile 1 is the own-FFT plan over `split` graph
+    branches (0 = 8)."""
     arr = pair_array(pairs)
-    opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, 0, 0, 0)
+    opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, tile, split, 0)
     check(lib().xc_batch_f32(arr, len(pairs), _ptr(left_flat), _ptr(right_flat), _ptr(out_flat),
                              ctypes.byref(opt), _stream(stream)))
     return out_flat

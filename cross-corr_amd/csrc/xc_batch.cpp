@@ -49,7 +49,9 @@ struct xc_batch_plan {
     std::vector<hipEvent_t> joins;
 
     ~xc_batch_plan() {
-        // work launched from this plan may still be queued: let the device drain first
+        // work launched from this plan may still be queued: let its device drain first
+        int cur = 0;
+        const bool other = hipGetDevice(&cur) == hipSuccess && cur != device && hipSetDevice(device) == hipSuccess;
         (void)hipDeviceSynchronize();
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -60,6 +62,7 @@ struct xc_batch_plan {
         for (void* p : {(void*)d_items, (void*)d_pairs, (void*)d_tables, (void*)d_counters})
             if (p) (void)hipFree(p);
         for (void* p : allocations) (void)hipFree(p);
+        if (other) (void)hipSetDevice(cur);
     }
 };
 
@@ -553,8 +556,10 @@ void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, con
     }
     static std::mutex mu;
     static std::list<std::pair<std::string, std::unique_ptr<xc_batch_plan>>> cache;
+    // per stream too: a plan's workspaces and work counter serve one execution at a time
     std::string key(reinterpret_cast<const char*>(&o), sizeof(o));
     key.append(pairs_key(pairs, num_pairs));
+    key.append(reinterpret_cast<const char*>(&s), sizeof(s));
     std::lock_guard<std::mutex> lock(mu);
     xc_batch_plan* plan = nullptr;
     for (auto it = cache.begin(); it != cache.end(); ++it)

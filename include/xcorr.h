@@ -135,9 +135,9 @@ xc_status xc_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* le
  *    by several pairs is transformed once), classes dealt largest first to opt->split
  *    parallel graph branches (1..8, 0 = 8), each with its own workspaces; re-captured when
  *    the matrix pointers change.  rows, cols <= 2048.
- * xc_batch_f32 keeps a few such plans cached by pair list.  A plan must not be executed
- * concurrently with itself (its work counter and workspaces are shared); destroy
- * synchronises the device. */
+ * xc_batch_f32 keeps a few such plans cached by (pair list, device, stream).  A plan must not
+ * be executed concurrently with itself (its work counter and workspaces are shared); destroy
+ * synchronises the plan's device. */
 typedef struct xc_batch_plan xc_batch_plan;
 xc_status xc_batch_plan_create_f32(const xc_pair* pairs, uint32_t num_pairs, const xc_options* opt,
                                    xc_batch_plan** plan);

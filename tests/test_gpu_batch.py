@@ -252,3 +252,40 @@ def test_batch_fft_plan_ragged_class_of_one_padded_size():
     plan.close()
     assert not np.isnan(got).any()
     batch_oracle_check(lf, rf, pairs, got, 1e-6)
+
+
+@pytest.mark.parametrize("prec", ["fft64k", "bf16x3"])
+def test_batch_on_two_streams_at_once(prec):
+    """xc_batch_f32 caches its plans per (pair list, device, stream): the same pair list run
+    on two streams at the same time, with different data, gives each stream its own
+    single-stream results (bit for bit: the kernels are deterministic)."""
+    x = xcorr()
+    lf, rf, pairs, olen, _ = uneven_batch(0, 6, sizes=[20, 33, 64, 90, 129])
+    arr = x.pair_array(pairs)
+    data = [(dev(lf * (1.0 + i)), dev(rf[::-1].copy() if i else rf)) for i in range(2)]
+
+    def run(L, R, out):
+        if prec == "fft64k":  # the own-FFT plan over two graph branches
+            x.batch(arr, L, R, out, precision="fft64", tile=1, split=2)
+        else:
+            x.batch(arr, L, R, out, precision="bf16x3")
+    want = []
+    for L, R in data:
+        o = torch.empty(olen, device="cuda")
+        run(L, R, o)
+        want.append(o)
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    got = [[], []]
+    print(f"batch on two streams, {prec}", flush=True)
+    for _ in range(3):
+        for i in range(2):
+            with torch.cuda.stream(streams[i]):
+                o = torch.empty(olen, device="cuda")
+                run(*data[i], o)
+                got[i].append(o)
+    torch.cuda.synchronize()
+    for i in range(2):
+        for g in got[i]:
+            assert torch.equal(g, want[i])
+    batch_oracle_check(lf, rf, pairs, want[0].cpu().numpy(), 2e-6 if prec == "bf16x3" else 1e-6)
