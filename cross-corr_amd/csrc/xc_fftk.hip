@@ -40,7 +40,7 @@
 namespace xc {
 namespace {
 
-constexpr int kFT = 256;              // threads per block
+constexpr int kFT = 256;              // threads per block: slab kernels, block-owned column FFTs
 constexpr int kLogMaxP = 12;          // longest transform: 4096 (the twiddle table's length)
 constexpr int kSlab = 1 << kLogMaxP;  // twiddle table entries
 
