@@ -125,7 +125,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this workload runs on one GPU; --gpus {args.gpus} is not supported "
-              "(pairs are independent; the sharding design is described in DESIGN.md)", file=sys.stderr)
+              "(pairs are independent: a multi-GPU run would deal pairs to GPUs, one process each)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
