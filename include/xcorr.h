@@ -40,7 +40,7 @@ typedef enum xc_status {
     XC_ERR_INVALID_ARGUMENT = 1, /* bad size, count, tuning value or family */
     XC_ERR_HIP = 2,              /* a HIP runtime call failed */
     XC_ERR_FFT = 3,              /* a hipFFT/rocFFT call failed */
-    XC_ERR_UNSUPPORTED = 4,      /* valid request this build cannot serve */
+    XC_ERR_UNSUPPORTED = 4,      /* valid request the chosen method cannot serve */
     XC_ERR_NO_MEMORY = 5         /* workspace allocation failed */
 } xc_status;
 
