@@ -64,6 +64,8 @@ def kernel_bytes(name, H, W, n, m, k):
     """Algorithmic HBM bytes of one launch of a kernel of the FFT methods: every array the
     kernel must read is read once and every array it must write is written once."""
     Pr, Pc = max(16, pow2ceil(2 * H - 1)), max(16, pow2ceil(2 * W - 1))
+    # rows: 3 * 2^n (n = 4..9) when shorter than the power of two (fftk_row_len)
+    Pc = min([Pc] + [3 * 2 ** e for e in range(4, 10) if 3 * 2 ** e >= 2 * W - 1])
     Kc = Pc // 2 + 1
     SY, SX = 2 * H - 1, 2 * W - 1
     out = k * SY * SX * 4

@@ -103,8 +103,8 @@ void* plan_alloc(xc_batch_plan* plan, size_t bytes) {
     return p;
 }
 
-// Own-FFT plan: pairs grouped into classes of equal padded lengths (Pr x Pc, the powers of two
-// >= 2 rows - 1 and 2 cols - 1), one job per class whose matrices keep their own sizes (the
+// Own-FFT plan: pairs grouped into classes of equal transform lengths (Pr x Pc, fftk_col_len and
+// fftk_row_len), one job per class whose matrices keep their own sizes (the
 // job's per-matrix size table); per class the distinct left and right matrices (by offset and
 // size) are transformed once and every pair of the class is one result of the job.  Classes
 // are dealt to `branches` (1..8) parallel graph branches.
@@ -124,17 +124,12 @@ xc_batch_plan* fft_plan_create(const xc_pair* pairs, uint32_t num_pairs, uint32_
         uint32_t H = 0, W = 0;
         double work = 0.0;
     };
-    auto pad = [](uint32_t v) {
-        uint32_t p = 16;
-        while (p < 2 * v - 1) p <<= 1;
-        return p;
-    };
     std::map<std::pair<uint32_t, uint32_t>, cls> classes;
     for (uint32_t i = 0; i < num_pairs; ++i) {
         const xc_pair& p = pairs[i];
         if (2 * p.rows - 1 > 4096 || 2 * p.cols - 1 > 4096)
             fail(XC_ERR_UNSUPPORTED, "pair " + std::to_string(i) + ": the FFT batch plan takes rows, cols <= 2048");
-        cls& c = classes[{pad(p.rows), pad(p.cols)}];
+        cls& c = classes[{fftk_col_len(p.rows), fftk_row_len(p.cols)}];
         auto li = c.lidx.emplace(mkey{p.left_off, p.rows, p.cols}, (uint32_t)c.loffs.size());
         if (li.second) {
             c.loffs.push_back(p.left_off);

@@ -822,6 +822,226 @@ __global__ __launch_bounds__(NT) void rows_inv_kernel(const double2* __restrict_
     }
 }
 
+// Row transforms of P = 3N points (N = 2^LOGN, 16..512), for widths where 3N >= 2W - 1 is
+// shorter than the power of two.  Each of a block's T transforms (row pairs) is split into
+// the three sub-sequences x[3n + m] (m = 0, 1, 2) of N points, which go through fft_regs<LOGN>
+// as 3T wave-owned transforms (sub-transform s = 3t + m); a radix-3 pass through LDS combines
+// them (decimation in time): with F_m the sub-spectra and w = exp(-2 pi i / P),
+//   X[k + N j] = F_0[k] + W_3^j w^k F_1[k] + W_3^2j w^2k F_2[k],  j = 0, 1, 2.
+// Blocks of 1.5 N threads hold 4 transforms (8 rows, a whole 128-byte line of each column of
+// S or Z) from 128 points on, and 192 threads below (more rows per block).
+template <int LOGN>
+struct rows3 {
+    static constexpr int N = 1 << LOGN, P = 3 * N, Q = N / 8;
+    static constexpr int NT = N >= 128 ? 3 * N / 2 : 192;
+    static constexpr int T = 8 * NT / P;                    // transforms (row pairs) per block
+    static constexpr int LOGT = T == 4 ? 2 : (T == 8 ? 3 : (T == 16 ? 4 : 5));
+    static constexpr int XS = P + P / 8 + 1;                // LDS stride of a whole transform
+    static constexpr int SLOTS = 3 * T * (N + N / 8 + 1) > T * XS ? 3 * T * (N + N / 8 + 1) : T * XS;
+    static constexpr int NI = (T * N + NT - 1) / NT;        // radix-3 butterflies per thread
+    static_assert(NT % 64 == 0 && (1 << LOGT) == T && 8 * NT == T * P, "rows3 geometry");
+};
+template <int LOGN>
+__device__ __forceinline__ int xat(int t, int p) {
+    return t * rows3<LOGN>::XS + p + (p >> 3);
+}
+
+// tw3[j] = exp(-2 pi i j / 12288): the P-th roots for P = 3 * 2^n <= 12288, one table per
+// device, kept for the process.
+constexpr int kTw3 = 3 * kSlab;
+const double2* twiddles3();
+
+// Radix-3 butterfly of the combine pass: (a, b, c) = (F_0, w^k F_1, w^2k F_2) -> X[k + N j].
+__device__ __forceinline__ void dft3(const double2 a, const double2 b, const double2 c, double2& x0, double2& x1,
+                                     double2& x2) {
+    constexpr double h = 0.86602540378443864676;  // sqrt(3) / 2
+    const double2 sm = cadd(b, c), df = csub(b, c);
+    const double2 t1 = make_double2(a.x - 0.5 * sm.x, a.y - 0.5 * sm.y);
+    const double2 t2 = make_double2(h * df.y, -h * df.x);  // -i sqrt(3)/2 (b - c)
+    x0 = cadd(a, sm);
+    x1 = cadd(t1, t2);
+    x2 = csub(t1, t2);
+}
+
+// The radix-3 pass of the block's T transforms: sub-spectra F_m[k] at at<LOGN>(3t + m, k) in
+// buf -> X[t][k + N j] in registers (this thread's butterflies idx = threadIdx + NT q,
+// t = idx / N, k = idx mod N).
+template <int LOGN>
+__device__ __forceinline__ void radix3_pass(const double2* buf, const double2* __restrict__ tw3,
+                                            double2 (&X)[rows3<LOGN>::NI][3]) {
+    using G = rows3<LOGN>;
+#pragma unroll
+    for (int q = 0; q < G::NI; ++q) {
+        const int idx = (int)threadIdx.x + G::NT * q;
+        const int tt = idx >> LOGN, k = idx & (G::N - 1);
+        if (idx < G::T * G::N) {
+            const double2 f0 = buf[at<LOGN>(3 * tt, k)];
+            const double2 f1 = cmul(tw3[k * (kTw3 / G::P)], buf[at<LOGN>(3 * tt + 1, k)]);
+            const double2 f2 = cmul(tw3[2 * k * (kTw3 / G::P)], buf[at<LOGN>(3 * tt + 2, k)]);
+            dft3(f0, f1, f2, X[q][0], X[q][1], X[q][2]);
+        }
+    }
+}
+
+// rows_fwd for P = 3N: block (matrix, group of 2T rows r0 = 2T g ...), otherwise as
+// rows_fwd_slab_kernel (matrices, offsets, sizes, output S).
+template <typename Tin, int LOGN>
+__global__ __launch_bounds__(rows3<LOGN>::NT) void rows_fwd3_kernel(
+    const Tin* __restrict__ XL, const Tin* __restrict__ XR, const uint64_t* __restrict__ moff,
+    const uint2* __restrict__ mdims, uint32_t nl, uint32_t H, uint32_t W, uint32_t groups, double2* __restrict__ SL,
+    double2* __restrict__ SR, const double2* __restrict__ tw, const double2* __restrict__ tw3) {
+    using G = rows3<LOGN>;
+    constexpr int N = G::N, P = G::P, Q = G::Q, T = G::T;
+    constexpr uint32_t Kc = P / 2 + 1;
+    extern __shared__ double2 buf[];
+    const uint32_t matg = blockIdx.x / groups, g = blockIdx.x - matg * groups;
+    const bool left = matg < nl;
+    const uint32_t mat = left ? matg : matg - nl;
+    uint32_t Hm = H, Wm = W;
+    if (mdims) {
+        const uint2 d = mdims[matg];
+        Hm = d.x;
+        Wm = d.y;
+    }
+    const uint32_t r0 = g * (2 * T);
+    if (r0 >= Hm) return;  // a row group beyond a smaller matrix
+    const Tin* Xm = (left ? XL : XR) + (moff ? moff[matg] : (size_t)mat * H * W);
+    const regs_pos<LOGN> ps;
+    const int sub = ps.t, tt = sub / 3, m = sub - 3 * tt, i = ps.i;
+    const uint32_t a = r0 + 2 * tt, b = a + 1;
+    const bool oka = a < Hm, okb = b < Hm;
+    const Tin* xa = Xm + (oka ? (size_t)a * Wm : 0);
+    const Tin* xb = Xm + (okb ? (size_t)b * Wm : 0);
+    const regs_roots<LOGN> rr = load_roots<LOGN>(i, tw);
+    Tin va[8], vb[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {  // element n = i + Q r of sub-sequence m: column 3n + m
+        const uint32_t c = 3u * (uint32_t)(i + Q * r) + (uint32_t)m, cc = c < Wm ? c : Wm - 1;
+        va[r] = xa[cc];
+        vb[r] = xb[cc];
+    }
+    double2 u[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const bool in = 3u * (uint32_t)(i + Q * r) + (uint32_t)m < Wm;
+        u[r] = make_double2(oka && in ? (double)va[r] : 0.0, okb && in ? (double)vb[r] : 0.0);
+    }
+    fft_regs<LOGN>(u, buf + at<LOGN>(ps.t0, 0), rr, sub - ps.t0, i);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) buf[at<LOGN>(sub, i + Q * r)] = u[r];
+    __syncthreads();
+    double2 X[G::NI][3];
+    radix3_pass<LOGN>(buf, tw3, X);
+    __syncthreads();  // X overwrites the sub-spectra
+#pragma unroll
+    for (int q = 0; q < G::NI; ++q) {
+        const int idx = (int)threadIdx.x + G::NT * q;
+        if (idx < T * N) {
+            const int t2 = idx >> LOGN, k = idx & (N - 1);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) buf[xat<LOGN>(t2, k + N * j)] = X[q][j];
+        }
+    }
+    __syncthreads();
+    // split into the two rows' half spectra (as split_rows), column-major into S
+    double2* Sm = (left ? SL : SR) + (size_t)mat * Kc * H;
+    for (int idx = threadIdx.x; idx < (int)Kc * 2 * T; idx += G::NT) {
+        const int k = idx >> (G::LOGT + 1), rw = idx & (2 * T - 1);
+        const uint32_t r = r0 + rw;
+        if (r >= Hm) continue;
+        const int t2 = rw >> 1;
+        const double2 zk = buf[xat<LOGN>(t2, k)];
+        const double2 zn = cconj(buf[xat<LOGN>(t2, k == 0 ? 0 : P - k)]);
+        double2 v;
+        if ((rw & 1) == 0) {
+            v = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y + zn.y));
+        } else {
+            const double2 d = mul_mi(csub(zk, zn));
+            v = make_double2(0.5 * d.x, 0.5 * d.y);
+        }
+        Sm[(size_t)k * H + r] = v;
+    }
+}
+
+// rows_inv for P = 3N: block (pair item, group of 2T rows z0 = 2T g ... of Z), otherwise as
+// rows_inv_kernel (Z rows by z_row, output rows out_row_of_z, crop and quadrant swap by index).
+template <typename Tout, int LOGN>
+__global__ __launch_bounds__(rows3<LOGN>::NT) void rows_inv3_kernel(
+    const double2* __restrict__ Z, Tout* __restrict__ O, const uint64_t* __restrict__ ooff,
+    const uint2* __restrict__ ptab, const uint2* __restrict__ mdims, uint32_t nl, uint32_t q0, uint32_t H, uint32_t W,
+    uint32_t Ys, uint32_t groups, double scale, const double2* __restrict__ tw, const double2* __restrict__ tw3) {
+    using G = rows3<LOGN>;
+    constexpr int N = G::N, P = G::P, Q = G::Q, T = G::T;
+    constexpr uint32_t Kc = P / 2 + 1;
+    extern __shared__ double2 buf[];
+    const uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t item = bid / groups, g = bid - item * groups;
+    if (mdims) {
+        const uint2 d = mdims[nl + ptab[q0 + item].y];
+        H = d.x;
+        W = d.y;
+    }
+    const uint32_t SX = 2 * W - 1;
+    const uint32_t z0 = g * (2 * T);
+    if (z0 >= 2 * H) return;
+    const double2* Zq = Z + (size_t)item * Kc * Ys;
+    // Hermitian extension of the two rows of each transform (conjugated, as rows_inv_kernel)
+    {
+        constexpr int NQ = ((int)Kc * T + G::NT - 1) / G::NT;
+        double2 A[NQ], B[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int idx = threadIdx.x + G::NT * q;
+            const int kk = idx >> G::LOGT, t = idx & (T - 1);
+            const uint32_t za = z0 + 2 * t;
+            const double2* zc = Zq + (size_t)(idx < (int)Kc * T ? kk : 0) * Ys + za;
+            A[q] = make_double2(0.0, 0.0);
+            B[q] = make_double2(0.0, 0.0);
+            if (idx < (int)Kc * T && out_row_of_z(za, H) >= 0) A[q] = zc[0];
+            if (idx < (int)Kc * T && out_row_of_z(za + 1, H) >= 0) B[q] = zc[1];
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int idx = threadIdx.x + G::NT * q;
+            if (idx >= (int)Kc * T) continue;
+            const int kk = idx >> G::LOGT, t = idx & (T - 1);
+            buf[xat<LOGN>(t, kk)] = cadd(cconj(A[q]), mul_mi(cconj(B[q])));
+            if (kk > 0 && kk < P / 2) buf[xat<LOGN>(t, P - kk)] = cadd(A[q], mul_mi(B[q]));
+        }
+    }
+    __syncthreads();
+    const regs_pos<LOGN> ps;
+    const int sub = ps.t, tt = sub / 3, m = sub - 3 * tt, i = ps.i;
+    const regs_roots<LOGN> rr = load_roots<LOGN>(i, tw);
+    double2 u[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) u[r] = buf[xat<LOGN>(tt, 3 * (i + Q * r) + m)];
+    __syncthreads();  // the sub-transforms' slots overlap the extended rows
+    fft_regs<LOGN>(u, buf + at<LOGN>(ps.t0, 0), rr, sub - ps.t0, i);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) buf[at<LOGN>(sub, i + Q * r)] = u[r];
+    __syncthreads();
+    double2 X[G::NI][3];
+    radix3_pass<LOGN>(buf, tw3, X);
+    // IFFT(Y) = conj(FFT(conj(Y))) / P: row A is the real part, row B minus the imaginary
+    Tout* Oq = O + (ooff ? ooff[q0 + item] : (size_t)(q0 + item) * (2 * H - 1) * SX);
+#pragma unroll
+    for (int q = 0; q < G::NI; ++q) {
+        const int idx = (int)threadIdx.x + G::NT * q;
+        if (idx >= T * N) continue;
+        const int t2 = idx >> LOGN, k = idx & (N - 1);
+        const int oa = out_row_of_z(z0 + 2 * t2, H), ob = out_row_of_z(z0 + 2 * t2 + 1, H);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int c = k + N * j;
+            const int ox = c <= (int)W - 1 ? c + (int)W - 1 : (c >= P - (int)W + 1 ? c - (P - (int)W + 1) : -1);
+            if (ox < 0) continue;
+            if (oa >= 0) Oq[(size_t)oa * SX + ox] = (Tout)(X[q][j].x * scale);
+            if (ob >= 0) Oq[(size_t)ob * SX + ox] = (Tout)(-X[q][j].y * scale);
+        }
+    }
+}
+
 int ilog2_ceil(uint32_t x) {
     int e = 0;
     while ((1u << e) < x) ++e;
@@ -863,6 +1083,36 @@ const double2* twiddles() {
             fail(XC_ERR_NO_MEMORY, "twiddle table allocation failed");
         }
         XC_HIP(hipMemcpy(d, h.data(), sizeof(double2) * kSlab, hipMemcpyHostToDevice));
+        tables[dev] = d;
+    }
+    return tables[dev];
+}
+
+const double2* twiddles3() {
+    static std::mutex mu;
+    static std::vector<double2*> tables;
+    int dev = 0;
+    XC_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)tables.size() <= dev) tables.resize(dev + 1, nullptr);
+    if (!tables[dev]) {
+        std::vector<double2> h(kTw3);
+        const double two_pi = 6.283185307179586476925286766559;
+        for (int j = 0; j < kTw3; ++j) {
+            const double a = two_pi * (double)j / (double)kTw3;
+            h[j] = make_double2(std::cos(a), -std::sin(a));
+        }
+        // exact at the twelfths of the circle (the radix-3 and radix-4 points)
+        const double s3 = 0.86602540378443864676;
+        const double c12[12][2] = {{1, 0},   {s3, -0.5}, {0.5, -s3}, {0, -1},  {-0.5, -s3}, {-s3, -0.5},
+                                   {-1, 0},  {-s3, 0.5}, {-0.5, s3}, {0, 1},   {0.5, s3},   {s3, 0.5}};
+        for (int q = 0; q < 12; ++q) h[q * (kTw3 / 12)] = make_double2(c12[q][0], c12[q][1]);
+        double2* d = nullptr;
+        if (hipMalloc(&d, sizeof(double2) * kTw3) != hipSuccess) {
+            (void)hipGetLastError();
+            fail(XC_ERR_NO_MEMORY, "twiddle table allocation failed");
+        }
+        XC_HIP(hipMemcpy(d, h.data(), sizeof(double2) * kTw3, hipMemcpyHostToDevice));
         tables[dev] = d;
     }
     return tables[dev];
@@ -915,7 +1165,9 @@ void allow_lds(K kernel, uint32_t bytes) {
 }
 
 struct fftk_geom {
-    int logPr, logPc;
+    int logPr, logPc;  // logPc: of a power-of-two row length
+    bool f3;           // row length Pc = 3 * 2^logN (rows3 kernels)
+    int logN;
     uint32_t Pr, Pc, Kc;
     slab_shape v1, vc, v3;
     uint32_t g1, cg, g3;
@@ -928,15 +1180,24 @@ fftk_geom geom_of(uint32_t H, uint32_t W) {
         fail(XC_ERR_UNSUPPORTED, "own-FFT method: rows and cols must be 1..2048");
     fftk_geom g{};
     g.logPr = std::max(4, ilog2_ceil(2 * H - 1));
-    g.logPc = std::max(4, ilog2_ceil(2 * W - 1));
     g.Pr = 1u << g.logPr;
-    g.Pc = 1u << g.logPc;
+    g.Pc = fftk_row_len(W);
+    g.f3 = (g.Pc & (g.Pc - 1)) != 0;
+    g.logN = g.f3 ? ilog2_ceil(g.Pc / 3) : 0;
+    g.logPc = g.f3 ? 0 : ilog2_ceil(g.Pc);
     g.Kc = g.Pc / 2 + 1;
     // Slab shape per kernel (see shape_for): rows_fwd 2T input rows per block, cols C columns,
     // rows_inv 2T output rows.
-    g.v1 = shape_for(g.logPc, (H + 1) / 2, rows_threads(g.logPc, false));
     g.vc = shape_for(g.logPr, g.Kc, cols_threads(g.logPr));
-    g.v3 = shape_for(g.logPc, H, rows_threads(g.logPc, true));
+    if (g.f3) {  // rows3<logN>: T = 8 NT / P transforms per block of NT threads
+        const uint32_t n = g.Pc / 3, nt = n >= 128 ? 3 * n / 2 : 192, t = 8 * nt / g.Pc;
+        const uint32_t slots = std::max(3 * t * (n + n / 8 + 1), t * (g.Pc + g.Pc / 8 + 1));
+        g.v1 = slab_shape{0, true, ilog2_ceil(t), slots * (uint32_t)sizeof(double2)};
+        g.v3 = g.v1;
+    } else {
+        g.v1 = shape_for(g.logPc, (H + 1) / 2, rows_threads(g.logPc, false));
+        g.v3 = shape_for(g.logPc, H, rows_threads(g.logPc, true));
+    }
     g.g1 = (H + (2u << g.v1.logT) - 1) >> (g.v1.logT + 1);
     g.cg = (g.Kc + (1u << g.vc.logT) - 1) >> g.vc.logT;
     g.g3 = (2 * H + (2u << g.v3.logT) - 1) >> (g.v3.logT + 1);  // a block may be taller
@@ -1000,6 +1261,21 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
     const uint32_t chunk = std::max<uint32_t>(1, std::min<uint32_t>(j.pairs, ws.zchunk));
     const uint2* ptab = reinterpret_cast<const uint2*>(j.ptab);
     const uint2* mdims = reinterpret_cast<const uint2*>(j.mdims);
+    const double2* tw3 = g.f3 ? twiddles3() : nullptr;
+    if (g.f3) {
+        with_logp(g.logN, [&](auto LN) {
+            constexpr int LOGN = decltype(LN)::value;
+            if constexpr (LOGN <= 9) {
+                using G3 = rows3<LOGN>;
+                if (G3::SLOTS * sizeof(double2) != g.v1.lds || G3::P != (int)g.Pc)
+                    fail(XC_ERR_UNSUPPORTED, "own-FFT method: inconsistent 3N-point row geometry");
+                auto k = rows_fwd3_kernel<Tio, LOGN>;
+                allow_lds(k, g.v1.lds);
+                launch_timed("fftk_rows_fwd", timing, k, dim3((j.nl + j.nr) * g.g1), dim3(G3::NT), g.v1.lds, s, L, R,
+                             j.moff, mdims, j.nl, H, W, g.g1, SL, SR, tw, tw3);
+            }
+        });
+    } else
     with_logp(g.logPc, [&](auto LP) {
         constexpr int LOGPC = decltype(LP)::value;
         const uint32_t items = (j.nl + j.nr) * g.g1;
@@ -1031,6 +1307,20 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
     for (uint32_t q0 = 0; q0 < j.pairs; q0 += chunk) {
         const uint32_t nq = std::min(chunk, j.pairs - q0);
         cols("fftk_cols", SR, Zw, std::integral_constant<int, 1>{}, nq, q0);
+        if (g.f3) {
+            with_logp(g.logN, [&](auto LN) {
+                constexpr int LOGN = decltype(LN)::value;
+                if constexpr (LOGN <= 9) {
+                    using G3 = rows3<LOGN>;
+                    auto k = rows_inv3_kernel<Tio, LOGN>;
+                    allow_lds(k, g.v3.lds);
+                    launch_timed("fftk_rows_inv", timing, k, dim3(nq * g.g3), dim3(G3::NT), g.v3.lds, s,
+                                 (const double2*)Zw, O, j.ooff, ptab, mdims, j.nl, q0, H, W, g.Ys, g.g3, scale, tw,
+                                 tw3);
+                }
+            });
+            continue;
+        }
         with_logp(g.logPc, [&](auto LP) {
             auto k = rows_inv_kernel<Tio, decltype(LP)::value>;
             allow_lds(k, g.v3.lds);
@@ -1040,7 +1330,8 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
         });
     }
     XC_HIP(hipGetLastError());
-    record_launch("fftk_rows_inv", dim3(g.g3), dim3(rows_threads(g.logPc, true)), g.v3.lds, H, W, j.pairs, 1);
+    record_launch("fftk_rows_inv", dim3(g.g3), dim3(g.f3 ? (g.Pc / 3 >= 128 ? g.Pc / 2 : 192) : rows_threads(g.logPc, true)),
+                  g.v3.lds, H, W, j.pairs, 1);
 }
 
 template <typename Tio>
@@ -1066,6 +1357,19 @@ void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, hipStrea
 
 }  // namespace
 
+uint32_t fftk_col_len(uint32_t rows) {
+    uint32_t p = 16;
+    while (p < 2 * rows - 1) p <<= 1;
+    return p;
+}
+
+uint32_t fftk_row_len(uint32_t cols) {
+    const uint32_t p = fftk_col_len(cols);
+    for (uint32_t n = 16; n <= 512; n <<= 1)
+        if (3 * n >= 2 * cols - 1) return 3 * n < p ? 3 * n : p;
+    return p;
+}
+
 fftk_ws_sizes fftk_sizes(const fftk_job& j) {
     const fftk_geom g = geom_of(j.H, j.W);
     fftk_ws_sizes z{};
@@ -1078,7 +1382,10 @@ fftk_ws_sizes fftk_sizes(const fftk_job& j) {
     return z;
 }
 
-void fftk_prepare() { (void)twiddles(); }
+void fftk_prepare() {
+    (void)twiddles();
+    (void)twiddles3();
+}
 
 void fftk_launch_f32(const fftk_job& j, const float* L, const float* R, float* O, const fftk_ws& ws, hipStream_t s) {
     launch_job<float>(j, L, R, O, ws, s, false);

@@ -82,6 +82,10 @@ struct fftk_ws {
     uint32_t zchunk;
 };
 fftk_ws_sizes fftk_sizes(const fftk_job& j);
+// Transform lengths of the own-FFT method: columns the power of two >= 2 rows - 1 (at least
+// 16); rows that, or 3 * 2^n (n = 4..9) when that is shorter.  Matrices of one job share them.
+uint32_t fftk_col_len(uint32_t rows);
+uint32_t fftk_row_len(uint32_t cols);
 void fftk_prepare();  // device tables the kernels read (call before stream capture)
 void fftk_launch_f32(const fftk_job& j, const float* L, const float* R, float* O, const fftk_ws& ws, hipStream_t s);
 

@@ -59,7 +59,7 @@ CASES = [
     ("one_to_one", 9, 17, 1, 1, "u01"),       # padded 32 (rows 17..32 of the column FFT are zero)
     ("one_to_one", 128, 128, 1, 1, "u01"),
     ("one_to_one", 313, 125, 1, 1, "u01"),
-    ("one_to_one", 3, 600, 1, 1, "u11"),      # 2048-point rows, 16-point columns
+    ("one_to_one", 3, 600, 1, 1, "u11"),      # 1536-point rows (3 x 512), 16-point columns
     ("one_to_one", 600, 3, 1, 1, "u11"),
     ("one_to_many", 33, 65, 1, 5, "u01"),
     ("one_to_many", 64, 64, 1, 12, "u11"),
@@ -67,11 +67,15 @@ CASES = [
     ("n_to_mn", 32, 32, 2, 6, "u01"),
     ("n_to_m", 64, 64, 4, 6, "normal"),
     ("n_to_m", 17, 23, 3, 5, "u11"),
+    ("one_to_many", 37, 40, 1, 4, "u11"),     # 96-point rows (3 x 32)
+    ("n_to_m", 50, 150, 2, 3, "normal"),      # 384-point rows (3 x 128)
     ("one_to_many", 129, 256, 1, 3, "u11"),   # 512-point register path: rows and columns
     ("n_to_m", 200, 140, 2, 2, "normal"),     # 512-point columns, 512-point rows, partial H
     ("one_to_one", 256, 70, 1, 1, "u01"),     # 512-point columns only
-    ("one_to_many", 140, 270, 1, 2, "u01"),   # 1024-point rows (8-row blocks), last row group partial
-    ("n_to_m", 45, 700, 2, 2, "u11"),         # 2048-point rows, 45 rows: 6 groups of 8, the last partial
+    ("one_to_many", 140, 270, 1, 2, "u01"),   # 768-point rows (3 x 256), last row group partial
+    ("n_to_m", 45, 700, 2, 2, "u11"),         # 1536-point rows, 45 rows: 6 groups of 8, the last partial
+    ("one_to_many", 100, 450, 1, 2, "u01"),   # 1024-point rows (8-row blocks)
+    ("n_to_m", 21, 800, 2, 2, "u11"),         # 2048-point rows, last row group partial
 ]
 
 
