@@ -828,18 +828,19 @@ __global__ __launch_bounds__(NT) void rows_inv_kernel(const double2* __restrict_
 // as 3T wave-owned transforms (sub-transform s = 3t + m); a radix-3 pass through LDS combines
 // them (decimation in time): with F_m the sub-spectra and w = exp(-2 pi i / P),
 //   X[k + N j] = F_0[k] + W_3^j w^k F_1[k] + W_3^2j w^2k F_2[k],  j = 0, 1, 2.
-// Blocks of 1.5 N threads hold 4 transforms (8 rows, a whole 128-byte line of each column of
-// S or Z) from 128 points on, and 192 threads below (more rows per block).
+// A block holds T transforms (3T N / 8 threads, a multiple of 64): 2 at 1536 points (two
+// blocks per CU measured faster than one), 4 from 384 to 768, more below (192 threads).
+constexpr int rows3_t(int logN) { return logN >= 9 ? 2 : (logN >= 7 ? 4 : 512 >> logN); }
 template <int LOGN>
 struct rows3 {
     static constexpr int N = 1 << LOGN, P = 3 * N, Q = N / 8;
-    static constexpr int NT = N >= 128 ? 3 * N / 2 : 192;
-    static constexpr int T = 8 * NT / P;                    // transforms (row pairs) per block
-    static constexpr int LOGT = T == 4 ? 2 : (T == 8 ? 3 : (T == 16 ? 4 : 5));
+    static constexpr int T = rows3_t(LOGN);                 // transforms (row pairs) per block
+    static constexpr int NT = 3 * T * N / 8;
+    static constexpr int LOGT = T == 2 ? 1 : (T == 4 ? 2 : (T == 8 ? 3 : (T == 16 ? 4 : 5)));
     static constexpr int XS = P + P / 8 + 1;                // LDS stride of a whole transform
     static constexpr int SLOTS = 3 * T * (N + N / 8 + 1) > T * XS ? 3 * T * (N + N / 8 + 1) : T * XS;
     static constexpr int NI = (T * N + NT - 1) / NT;        // radix-3 butterflies per thread
-    static_assert(NT % 64 == 0 && (1 << LOGT) == T && 8 * NT == T * P, "rows3 geometry");
+    static_assert(NT % 64 == 0 && NT <= 1024 && (1 << LOGT) == T && 8 * NT == T * P, "rows3 geometry");
 };
 template <int LOGN>
 __device__ __forceinline__ int xat(int t, int p) {
@@ -863,11 +864,31 @@ __device__ __forceinline__ void dft3(const double2 a, const double2 b, const dou
     x2 = csub(t1, t2);
 }
 
+// Twiddles w^k, w^2k of this thread's radix-3 butterflies (idx = threadIdx + NT q, k = idx mod
+// N), read at kernel start like the pass roots (a table read between two LDS phases would
+// wait on memory).
+template <int LOGN>
+struct r3_roots {
+    double2 w1[rows3<LOGN>::NI], w2[rows3<LOGN>::NI];
+};
+template <int LOGN>
+__device__ __forceinline__ r3_roots<LOGN> load_r3_roots(const double2* __restrict__ tw3) {
+    using G = rows3<LOGN>;
+    r3_roots<LOGN> w;
+#pragma unroll
+    for (int q = 0; q < G::NI; ++q) {
+        const int k = ((int)threadIdx.x + G::NT * q) & (G::N - 1);
+        w.w1[q] = tw3[k * (kTw3 / G::P)];
+        w.w2[q] = tw3[2 * k * (kTw3 / G::P)];
+    }
+    return w;
+}
+
 // The radix-3 pass of the block's T transforms: sub-spectra F_m[k] at at<LOGN>(3t + m, k) in
 // buf -> X[t][k + N j] in registers (this thread's butterflies idx = threadIdx + NT q,
 // t = idx / N, k = idx mod N).
 template <int LOGN>
-__device__ __forceinline__ void radix3_pass(const double2* buf, const double2* __restrict__ tw3,
+__device__ __forceinline__ void radix3_pass(const double2* buf, const r3_roots<LOGN>& w,
                                             double2 (&X)[rows3<LOGN>::NI][3]) {
     using G = rows3<LOGN>;
 #pragma unroll
@@ -876,8 +897,8 @@ __device__ __forceinline__ void radix3_pass(const double2* buf, const double2* _
         const int tt = idx >> LOGN, k = idx & (G::N - 1);
         if (idx < G::T * G::N) {
             const double2 f0 = buf[at<LOGN>(3 * tt, k)];
-            const double2 f1 = cmul(tw3[k * (kTw3 / G::P)], buf[at<LOGN>(3 * tt + 1, k)]);
-            const double2 f2 = cmul(tw3[2 * k * (kTw3 / G::P)], buf[at<LOGN>(3 * tt + 2, k)]);
+            const double2 f1 = cmul(w.w1[q], buf[at<LOGN>(3 * tt + 1, k)]);
+            const double2 f2 = cmul(w.w2[q], buf[at<LOGN>(3 * tt + 2, k)]);
             dft3(f0, f1, f2, X[q][0], X[q][1], X[q][2]);
         }
     }
@@ -913,6 +934,7 @@ __global__ __launch_bounds__(rows3<LOGN>::NT) void rows_fwd3_kernel(
     const Tin* xa = Xm + (oka ? (size_t)a * Wm : 0);
     const Tin* xb = Xm + (okb ? (size_t)b * Wm : 0);
     const regs_roots<LOGN> rr = load_roots<LOGN>(i, tw);
+    const r3_roots<LOGN> w3 = load_r3_roots<LOGN>(tw3);
     Tin va[8], vb[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) {  // element n = i + Q r of sub-sequence m: column 3n + m
@@ -931,7 +953,7 @@ __global__ __launch_bounds__(rows3<LOGN>::NT) void rows_fwd3_kernel(
     for (int r = 0; r < 8; ++r) buf[at<LOGN>(sub, i + Q * r)] = u[r];
     __syncthreads();
     double2 X[G::NI][3];
-    radix3_pass<LOGN>(buf, tw3, X);
+    radix3_pass<LOGN>(buf, w3, X);
     __syncthreads();  // X overwrites the sub-spectra
 #pragma unroll
     for (int q = 0; q < G::NI; ++q) {
@@ -985,6 +1007,10 @@ __global__ __launch_bounds__(rows3<LOGN>::NT) void rows_inv3_kernel(
     const uint32_t z0 = g * (2 * T);
     if (z0 >= 2 * H) return;
     const double2* Zq = Z + (size_t)item * Kc * Ys;
+    const regs_pos<LOGN> ps;
+    const int sub = ps.t, tt = sub / 3, m = sub - 3 * tt, i = ps.i;
+    const regs_roots<LOGN> rr = load_roots<LOGN>(i, tw);
+    const r3_roots<LOGN> w3 = load_r3_roots<LOGN>(tw3);
     // Hermitian extension of the two rows of each transform (conjugated, as rows_inv_kernel)
     {
         constexpr int NQ = ((int)Kc * T + G::NT - 1) / G::NT;
@@ -1010,9 +1036,6 @@ __global__ __launch_bounds__(rows3<LOGN>::NT) void rows_inv3_kernel(
         }
     }
     __syncthreads();
-    const regs_pos<LOGN> ps;
-    const int sub = ps.t, tt = sub / 3, m = sub - 3 * tt, i = ps.i;
-    const regs_roots<LOGN> rr = load_roots<LOGN>(i, tw);
     double2 u[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) u[r] = buf[xat<LOGN>(tt, 3 * (i + Q * r) + m)];
@@ -1022,7 +1045,7 @@ __global__ __launch_bounds__(rows3<LOGN>::NT) void rows_inv3_kernel(
     for (int r = 0; r < 8; ++r) buf[at<LOGN>(sub, i + Q * r)] = u[r];
     __syncthreads();
     double2 X[G::NI][3];
-    radix3_pass<LOGN>(buf, tw3, X);
+    radix3_pass<LOGN>(buf, w3, X);
     // IFFT(Y) = conj(FFT(conj(Y))) / P: row A is the real part, row B minus the imaginary
     Tout* Oq = O + (ooff ? ooff[q0 + item] : (size_t)(q0 + item) * (2 * H - 1) * SX);
 #pragma unroll
@@ -1190,7 +1213,7 @@ fftk_geom geom_of(uint32_t H, uint32_t W) {
     // rows_inv 2T output rows.
     g.vc = shape_for(g.logPr, g.Kc, cols_threads(g.logPr));
     if (g.f3) {  // rows3<logN>: T = 8 NT / P transforms per block of NT threads
-        const uint32_t n = g.Pc / 3, nt = n >= 128 ? 3 * n / 2 : 192, t = 8 * nt / g.Pc;
+        const uint32_t n = g.Pc / 3, t = (uint32_t)rows3_t(g.logN);
         const uint32_t slots = std::max(3 * t * (n + n / 8 + 1), t * (g.Pc + g.Pc / 8 + 1));
         g.v1 = slab_shape{0, true, ilog2_ceil(t), slots * (uint32_t)sizeof(double2)};
         g.v3 = g.v1;
@@ -1330,7 +1353,8 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
         });
     }
     XC_HIP(hipGetLastError());
-    record_launch("fftk_rows_inv", dim3(g.g3), dim3(g.f3 ? (g.Pc / 3 >= 128 ? g.Pc / 2 : 192) : rows_threads(g.logPc, true)),
+    record_launch("fftk_rows_inv", dim3(g.g3),
+                  dim3(g.f3 ? 3u * (uint32_t)rows3_t(g.logN) * (g.Pc / 3) / 8 : (uint32_t)rows_threads(g.logPc, true)),
                   g.v3.lds, H, W, j.pairs, 1);
 }
 
