@@ -130,8 +130,8 @@ xc_status xc_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* le
  *    border-frame chunks for bf16x3) is built and uploaded once; each execution is one launch
  *    of the persistent kernel (kernel arguments are updated in the instantiated graph when
  *    the matrix pointers change).
- *  - XC_PREC_FFT64 tile 1: the own FFT kernels per class of equal padded lengths (pairs
- *    of different sizes with the same power-of-two padding share one job; a matrix shared
+ *  - XC_PREC_FFT64 tile 1: the own FFT kernels per class of equal transform lengths (pairs
+ *    of different sizes with the same padded lengths share one job; a matrix shared
  *    by several pairs is transformed once), classes dealt largest first to opt->split
  *    parallel graph branches (1..8, 0 = 8), each with its own workspaces; re-captured when
  *    the matrix pointers change.  rows, cols <= 2048.
