@@ -178,30 +178,53 @@ def result_shape(family, rows, cols, n, m):
     return (k, 2 * rows - 1, 2 * cols - 1)
 
 
+def _check_inputs(left, right):
+    """(n, rows, cols) and (m, rows, cols) contiguous GPU tensors of one dtype, float32 or
+    float64, on one device; a 2D tensor is one matrix.  Raises XcError otherwise."""
+    import torch
+    if not (isinstance(left, torch.Tensor) and isinstance(right, torch.Tensor)):
+        raise XcError(1, "inputs must be torch tensors")
+    if left.dim() == 2:
+        left = left.unsqueeze(0)
+    if right.dim() == 2:
+        right = right.unsqueeze(0)
+    if left.dim() != 3 or right.dim() != 3:
+        raise XcError(1, "inputs must be (count, rows, cols) tensors")
+    if left.dtype != right.dtype or left.dtype not in (torch.float32, torch.float64):
+        raise XcError(1, "inputs must both be float32 or both float64")
+    if tuple(right.shape[1:]) != tuple(left.shape[1:]):
+        raise XcError(1, "left and right matrices must have the same size")
+    if not (left.is_cuda and right.is_cuda):
+        raise XcError(1, "inputs must be GPU tensors")
+    if left.device != right.device:
+        raise XcError(1, "left and right matrices must be on the same device")
+    return left.contiguous(), right.contiguous()
+
+
+def _check_out(out, shape, like):
+    """A new result tensor, or the caller's `out` if it is a contiguous tensor of exactly
+    `shape` with the inputs' dtype on the inputs' device (XcError otherwise)."""
+    import torch
+    if out is None:
+        return torch.empty(shape, dtype=like.dtype, device=like.device)
+    if not isinstance(out, torch.Tensor) or out.device != like.device or out.dtype != like.dtype:
+        raise XcError(1, "out must be a tensor on the inputs' device with the inputs' dtype")
+    if tuple(out.shape) != tuple(shape) or not out.is_contiguous():
+        raise XcError(1, f"out must be a contiguous tensor of shape {tuple(shape)}")
+    return out
+
+
 def cross_corr(family, left, right, out=None, precision="auto", split=0, border=0, tile=0, stream=None):
     """Batched full cross-correlation of torch device tensors.
 
     left: (n, rows, cols), right: (m, rows, cols), float32 or float64, contiguous, on the GPU.
     Returns `out` of shape (k, 2*rows-1, 2*cols-1) in the family's result order.
     """
-    import torch
     fam = FAMILIES[family] if isinstance(family, str) else family
-    if left.dim() == 2:
-        left = left.unsqueeze(0)
-    if right.dim() == 2:
-        right = right.unsqueeze(0)
-    if not (left.is_cuda and right.is_cuda):
-        raise XcError(1, "inputs must be GPU tensors")
-    if left.dtype != right.dtype or left.dtype not in (torch.float32, torch.float64):
-        raise XcError(1, "inputs must both be float32 or both float64")
-    left, right = left.contiguous(), right.contiguous()
+    left, right = _check_inputs(left, right)
     n, rows, cols = left.shape
     m = right.shape[0]
-    if tuple(right.shape[1:]) != (rows, cols):
-        raise XcError(1, "left and right matrices must have the same size")
-    shape = result_shape(fam, rows, cols, n, m)
-    if out is None:
-        out = torch.empty(shape, dtype=left.dtype, device=left.device)
+    out = _check_out(out, result_shape(fam, rows, cols, n, m), left)
     opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, tile, split, border)
     fn = lib().xc_cross_corr_f32 if left.dtype == torch.float32 else lib().xc_cross_corr_f64
     check(fn(fam, _ptr(left), _ptr(right), _ptr(out), rows, cols, n, m, ctypes.byref(opt), _stream(stream)))
@@ -212,13 +235,12 @@ def fft_cross_corr(family, left, right, out=None, normalize=True, fp64_transform
     """FFT method; normalize=False returns the unnormalised (2*rows) x (2*cols) circular results."""
     import torch
     fam = FAMILIES[family] if isinstance(family, str) else family
-    left, right = left.contiguous(), right.contiguous()
+    left, right = _check_inputs(left, right)
     n, rows, cols = left.shape
     m = right.shape[0]
     k = result_shape(fam, rows, cols, n, m)[0]
     shape = (k, 2 * rows - 1, 2 * cols - 1) if normalize else (k, 2 * rows, 2 * cols)
-    if out is None:
-        out = torch.empty(shape, dtype=left.dtype, device=left.device)
+    out = _check_out(out, shape, left)
     if left.dtype == torch.float32:
         check(lib().xc_fft_cross_corr_f32(fam, _ptr(left), _ptr(right), _ptr(out), rows, cols, n, m,
                                           int(normalize), int(fp64_transform), _stream(stream)))

@@ -107,7 +107,7 @@ xc_status xc_cross_corr_f64(xc_family family, const double* left, const double* 
             xc::fail(XC_ERR_INVALID_ARGUMENT, "double data supports the direct (FP64) and FFT64 methods only");
         xc::family_args a{family, rows, cols, n, m, (uint32_t)xc::num_results(family, n, m)};
         if (o.precision == XC_PREC_FFT64 && o.tile == 1)
-            xc::run_fftk_f64(a, left, right, out, (hipStream_t)stream);
+            xc::run_fftk_f64(a, left, right, out, o.split, (hipStream_t)stream);
         else if (o.precision == XC_PREC_FFT64)
             xc::run_fft_f64(a, left, right, out, true, (hipStream_t)stream);
         else

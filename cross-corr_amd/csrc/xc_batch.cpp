@@ -341,7 +341,7 @@ xc_batch_plan* batch_plan_create(const xc_pair* pairs, uint32_t num_pairs, const
     XC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, plan->device));
     plan->grid = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)cus * batch_blocks_per_cu(), plan->n_items));
     plan->lds = batch_lds_bytes(kind);
-    XC_HIP(hipFuncSetAttribute(batch_kernel_ptr(kind), hipFuncAttributeMaxDynamicSharedMemorySize, plan->lds));
+    allow_dynamic_lds(batch_kernel_ptr(kind), plan->lds);
     XC_HIP(hipStreamCreateWithFlags(&plan->capture_stream, hipStreamNonBlocking));
     return plan.release();
 }

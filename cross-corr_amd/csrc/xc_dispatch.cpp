@@ -1,8 +1,11 @@
 // Method dispatch and device workspace management.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
+#include <set>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -23,6 +26,11 @@ std::mutex g_ws_mutex;
 // is never used by two calls at once; calls on different streams (or host threads with
 // their own streams) get their own sets.  Kept until the process ends.
 std::map<std::pair<int, hipStream_t>, std::vector<ws_slot>> g_ws;
+// Buffers replaced by a larger one.  They are never freed: another host thread may share
+// the stream and hold the old pointer without having enqueued its kernels yet, so no
+// synchronisation point here covers its work.  Each replacement at least doubles the
+// slot, so what is retired stays below the slot's final size.
+std::vector<void*> g_ws_retired;
 
 }  // namespace
 
@@ -34,21 +42,32 @@ void* workspace(int slot, size_t bytes, hipStream_t s) {
     if ((int)v.size() <= slot) v.resize(slot + 1);
     ws_slot& w = v[slot];
     if (w.bytes < bytes) {
-        if (w.ptr) {
-            // The old buffer may still be in use by queued work on this stream (and only
-            // this stream uses it).
-            XC_HIP(hipStreamSynchronize(s));
-            XC_HIP(hipFree(w.ptr));
-            w.ptr = nullptr;
-            w.bytes = 0;
-        }
-        if (hipMalloc(&w.ptr, bytes) != hipSuccess) {
+        size_t size = w.ptr ? std::max(bytes, 2 * w.bytes) : bytes;
+        void* p = nullptr;
+        if (hipMalloc(&p, size) != hipSuccess) {
             (void)hipGetLastError();
-            fail(XC_ERR_NO_MEMORY, "workspace allocation of " + std::to_string(bytes) + " bytes failed");
+            size = bytes;
+            if (hipMalloc(&p, size) != hipSuccess) {
+                (void)hipGetLastError();
+                fail(XC_ERR_NO_MEMORY, "workspace allocation of " + std::to_string(bytes) + " bytes failed");
+            }
         }
-        w.bytes = bytes;
+        if (w.ptr) g_ws_retired.push_back(w.ptr);
+        w.ptr = p;
+        w.bytes = size;
     }
     return w.ptr;
+}
+
+void allow_dynamic_lds(const void* kernel, uint32_t bytes) {
+    static std::mutex mu;
+    static std::set<std::tuple<const void*, int, uint32_t>> done;
+    int dev = 0;
+    XC_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if (done.count(std::make_tuple(kernel, dev, bytes))) return;
+    XC_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    done.emplace(kernel, dev, bytes);
 }
 
 void run_family_f32(const family_args& a, const float* L, const float* R, float* O, const xc_options& opt,
@@ -68,7 +87,7 @@ void run_family_f32(const family_args& a, const float* L, const float* R, float*
         return;
     case XC_PREC_FFT64:
         if (opt.tile == 1)
-            run_fftk_f32(a, L, R, O, s);
+            run_fftk_f32(a, L, R, O, opt.split, s);
         else if (opt.tile == 0)
             run_fft_f32(a, L, R, O, true, true, s);
         else

@@ -1182,9 +1182,7 @@ void with_logp(int logP, F&& f) {
 
 template <typename K>
 void allow_lds(K kernel, uint32_t bytes) {
-    if (bytes > 65536)
-        XC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   bytes));
+    if (bytes > 65536) allow_dynamic_lds(reinterpret_cast<const void*>(kernel), bytes);
 }
 
 struct fftk_geom {
@@ -1359,8 +1357,9 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
 }
 
 template <typename Tio>
-void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, hipStream_t s) {
+void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, uint32_t max_chunk, hipStream_t s) {
     fftk_job j{};
+    j.max_chunk = max_chunk;
     j.H = a.rows;
     j.W = a.cols;
     j.nl = a.family == XC_ONE_TO_ONE || a.family == XC_ONE_TO_MANY ? 1 : a.n;
@@ -1399,6 +1398,7 @@ fftk_ws_sizes fftk_sizes(const fftk_job& j) {
     fftk_ws_sizes z{};
     // Z is processed in chunks of pairs of at most ~2 GiB.
     z.chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(j.pairs, (size_t(2) << 30) / g.z_bytes));
+    if (j.max_chunk) z.chunk = std::min(z.chunk, j.max_chunk);
     z.sr = std::max<size_t>(g.s_bytes * j.nr, 16);
     z.sl = std::max<size_t>(g.s_bytes * j.nl, 16);
     z.lh = std::max<size_t>(g.col_bytes * j.nl, 16);
@@ -1415,12 +1415,13 @@ void fftk_launch_f32(const fftk_job& j, const float* L, const float* R, float* O
     launch_job<float>(j, L, R, O, ws, s, false);
 }
 
-void run_fftk_f32(const family_args& a, const float* L, const float* R, float* O, hipStream_t s) {
-    run_fftk<float>(a, L, R, O, s);
+void run_fftk_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t max_chunk, hipStream_t s) {
+    run_fftk<float>(a, L, R, O, max_chunk, s);
 }
 
-void run_fftk_f64(const family_args& a, const double* L, const double* R, double* O, hipStream_t s) {
-    run_fftk<double>(a, L, R, O, s);
+void run_fftk_f64(const family_args& a, const double* L, const double* R, double* O, uint32_t max_chunk,
+                  hipStream_t s) {
+    run_fftk<double>(a, L, R, O, max_chunk, s);
 }
 
 }  // namespace xc

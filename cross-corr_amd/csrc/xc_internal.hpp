@@ -55,6 +55,10 @@ struct timed {
     timed& operator=(const timed&) = delete;
 };
 
+// Raises the dynamic-LDS limit of `kernel` to `bytes` on the current device, once per
+// (kernel, device, size); thread-safe.
+void allow_dynamic_lds(const void* kernel, uint32_t bytes);
+
 // k-th result of a family -> (left index, right index); see the table in xcorr.h.
 __host__ __device__ inline void pair_of(int family, uint32_t n, uint32_t m, uint32_t k, uint32_t& l,
                                         uint32_t& r) {

@@ -72,6 +72,9 @@ struct fftk_job {
     // Uneven batches: (rows, cols) of every matrix (lefts, then rights), each at most H x W
     // and with the padded lengths of H x W; null when every matrix is H x W.
     const uint32_t* mdims = nullptr;
+    // Largest number of pairs per launch of the column and inverse-row kernels (0: as
+    // many as fit in 2 GiB of Z).
+    uint32_t max_chunk = 0;
 };
 struct fftk_ws_sizes {
     size_t sr, sl, lh, z;
@@ -90,8 +93,10 @@ void fftk_prepare();  // device tables the kernels read (call before stream capt
 void fftk_launch_f32(const fftk_job& j, const float* L, const float* R, float* O, const fftk_ws& ws, hipStream_t s);
 
 // FFT method on the library's own FP64 FFT kernels (xc_fftk.hip): rows, cols <= 2048.
-void run_fftk_f32(const family_args& a, const float* L, const float* R, float* O, hipStream_t s);
-void run_fftk_f64(const family_args& a, const double* L, const double* R, double* O, hipStream_t s);
+// max_chunk: pairs per Z chunk as fftk_job::max_chunk (0 = default).
+void run_fftk_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t max_chunk, hipStream_t s);
+void run_fftk_f64(const family_args& a, const double* L, const double* R, double* O, uint32_t max_chunk,
+                  hipStream_t s);
 
 void check_scatter(const void* src, const void* dst, uint32_t src_rows, uint32_t src_cols, uint32_t dst_rows,
                    uint32_t dst_cols, uint32_t dst_row, uint32_t dst_col);

@@ -560,12 +560,7 @@ void launch(const family_args& a, const float* L, const float* R, float* O, uint
     const uint32_t tiles_y = (SY + G::BY - 1) / G::BY;
     const uint64_t nblocks = (uint64_t)tiles_x * tiles_y * a.pairs;
     if (nblocks > 0x7fffffffULL) fail(XC_ERR_INVALID_ARGUMENT, "problem too large for one launch");
-    static bool attr_set = false;
-    if (!attr_set) {
-        XC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&mfma_kernel<KIND, WX, WY, KC, FLUSH, KAHAN, OCC>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, G::BYTES));
-        attr_set = true;
-    }
+    allow_dynamic_lds(reinterpret_cast<const void*>(&mfma_kernel<KIND, WX, WY, KC, FLUSH, KAHAN, OCC>), G::BYTES);
     dim3 grid((uint32_t)nblocks), block(256);
     record_launch(KIND == XC_PREC_BF16X3   ? "mfma_kernel<bf16x3>"
                   : KIND == XC_PREC_FP16X3 ? "mfma_kernel<fp16x3>"

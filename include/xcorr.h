@@ -87,7 +87,9 @@ typedef enum xc_precision {
 typedef struct xc_options {
     uint32_t precision;   /* xc_precision */
     uint32_t tile;        /* kernel-specific tile code from the tile catalog, 0 = default */
-    uint32_t split;       /* split of the reduction (overlap rows) across blocks, 0 = auto */
+    uint32_t split;       /* FP32: split of the reduction (overlap rows) across blocks;
+                             FFT64 tile 1: at most this many pairs per launch of the column
+                             and inverse-row kernels (less scratch memory); 0 = auto */
     uint32_t border;      /* BF16X3: overlap-area threshold for exact FP32 recompute, 0 = default */
     uint32_t reserved[4];
 } xc_options;

@@ -73,3 +73,44 @@ def test_reference_tuning_limits():
 
 def test_version():
     assert b"gfx950" in xcorr().lib().xc_version()
+
+
+def test_python_binding_rejects_bad_inputs_before_the_library():
+    # Host-side checks of cross_corr / fft_cross_corr: dtype, size, device and a
+    # caller-supplied `out` (shape, dtype, device, contiguity) are refused with XcError.
+    torch = pytest.importorskip("torch")
+    x = xcorr()
+    a = torch.zeros((1, 4, 5))
+    for fn in (x.cross_corr, x.fft_cross_corr):
+        with pytest.raises(x.XcError, match="float32 or both float64"):
+            fn("one_to_one", a.half(), a.half())
+        with pytest.raises(x.XcError, match="float32 or both float64"):
+            fn("one_to_one", a, a.double())
+        with pytest.raises(x.XcError, match="float32 or both float64"):
+            fn("one_to_one", a.int(), a.int())
+        with pytest.raises(x.XcError, match="same size"):
+            fn("one_to_one", a, torch.zeros((1, 4, 4)))
+        with pytest.raises(x.XcError, match="GPU tensors"):
+            fn("one_to_one", a, a)
+        with pytest.raises(x.XcError, match="torch tensors"):
+            fn("one_to_one", a.numpy(), a.numpy())
+        with pytest.raises(x.XcError, match=r"\(count, rows, cols\)"):
+            fn("one_to_one", a[None], a[None])
+
+
+def test_python_binding_checks_a_given_out():
+    torch = pytest.importorskip("torch")
+    x = xcorr()
+    like = torch.zeros((1, 4, 5))
+    shape = (1, 7, 9)
+    ok = torch.zeros(shape)
+    assert x._check_out(ok, shape, like) is ok
+    assert x._check_out(None, shape, like).shape == shape
+    with pytest.raises(x.XcError, match="shape"):
+        x._check_out(torch.zeros((1, 7, 8)), shape, like)
+    with pytest.raises(x.XcError, match="shape"):
+        x._check_out(torch.zeros((1, 9, 7)).transpose(1, 2), shape, like)
+    with pytest.raises(x.XcError, match="dtype"):
+        x._check_out(torch.zeros(shape, dtype=torch.float64), shape, like)
+    with pytest.raises(x.XcError, match="dtype"):
+        x._check_out(ok.numpy(), shape, like)
