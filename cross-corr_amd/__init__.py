@@ -35,7 +35,10 @@ class XcError(RuntimeError):
 
 class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_uint32), ("tile", ctypes.c_uint32), ("split", ctypes.c_uint32),
-                ("border", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+                ("border", ctypes.c_uint32), ("shifts_per_thread", ctypes.c_uint32),
+                ("right_mats_per_thread", ctypes.c_uint32), ("left_mats_per_thread", ctypes.c_uint32),
+                ("rows_per_step", ctypes.c_uint32), ("waves_per_block", ctypes.c_uint32),
+                ("rows_per_block", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
 
 
 class Pair(ctypes.Structure):
@@ -214,18 +217,34 @@ def _check_out(out, shape, like):
     return out
 
 
-def cross_corr(family, left, right, out=None, precision="auto", split=0, border=0, tile=0, stream=None):
+TILE_KEYS = ("shifts_per_thread", "right_mats_per_thread", "left_mats_per_thread", "rows_per_step",
+             "waves_per_block", "rows_per_block")
+
+
+def options(precision="auto", tile=0, split=0, border=0, **tiling):
+    """xc_options; `tiling` takes the direct kernel's TILE_KEYS (0 = library default)."""
+    opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, tile, split, border)
+    for key, value in tiling.items():
+        if key not in TILE_KEYS:
+            raise XcError(1, f"unknown option {key}")
+        setattr(opt, key, int(value))
+    return opt
+
+
+def cross_corr(family, left, right, out=None, precision="auto", split=0, border=0, tile=0, stream=None, **tiling):
     """Batched full cross-correlation of torch device tensors.
 
     left: (n, rows, cols), right: (m, rows, cols), float32 or float64, contiguous, on the GPU.
-    Returns `out` of shape (k, 2*rows-1, 2*cols-1) in the family's result order.
+    Returns `out` of shape (k, 2*rows-1, 2*cols-1) in the family's result order.  `tiling`
+    sets the direct kernel's register tile and schedule (TILE_KEYS, see xc_options).
     """
+    import torch
     fam = FAMILIES[family] if isinstance(family, str) else family
     left, right = _check_inputs(left, right)
     n, rows, cols = left.shape
     m = right.shape[0]
     out = _check_out(out, result_shape(fam, rows, cols, n, m), left)
-    opt = Options(PRECISIONS[precision] if isinstance(precision, str) else precision, tile, split, border)
+    opt = options(precision, tile, split, border, **tiling)
     fn = lib().xc_cross_corr_f32 if left.dtype == torch.float32 else lib().xc_cross_corr_f64
     check(fn(fam, _ptr(left), _ptr(right), _ptr(out), rows, cols, n, m, ctypes.byref(opt), _stream(stream)))
     return out

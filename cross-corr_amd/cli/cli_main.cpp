@@ -183,10 +183,13 @@ void pair_of(int family, uint32_t n, uint32_t m, uint64_t k, uint32_t& l, uint32
 }
 
 // Host computation of a family: naive_cpu_cross_corr (src/validate.hpp:94-132) per result,
-// accumulating in A, split over the host's threads.  Used by the cpu_* algorithms (A = T,
-// as the reference) and by `run -v` without a file (A = double).
+// accumulating in A.  The cpu_* algorithms run it as the reference does: A = T on one thread
+// (src/validate.hpp:236-239), so their timings compare with the reference's.  `run -v`
+// without a file validates against the same T-typed computation (src/cross_corr.hpp:90-97),
+// spread over the host's threads (`threads` = 0) since that is not timed.
 template <typename A, typename T>
-void cpu_family(int family, const matrix_array<T>& left, const matrix_array<T>& right, matrix_array<T>& out) {
+void cpu_family(int family, const matrix_array<T>& left, const matrix_array<T>& right, matrix_array<T>& out,
+                unsigned threads) {
     const int H = (int)left.rows, W = (int)left.cols;
     const int SX = 2 * W - 1;
     const uint64_t k = out.count;
@@ -207,7 +210,8 @@ void cpu_family(int family, const matrix_array<T>& left, const matrix_array<T>& 
                 }
         }
     };
-    const unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), (unsigned)k));
+    const unsigned avail = threads ? threads : std::thread::hardware_concurrency();
+    const unsigned nt = std::max(1u, std::min<unsigned>(avail, (unsigned)k));
     std::vector<std::thread> th;
     for (unsigned t = 0; t < nt; ++t) th.emplace_back(work, k * t / nt, k * (t + 1) / nt);
     for (auto& t : th) t.join();
@@ -408,15 +412,75 @@ resolved_args resolve_args(const alg_spec& alg, const json_args& js) {
     return r;
 }
 
+// Largest accepted value of a tuning key: the limits of the reference's release build
+// (CMakeLists.txt:16-45, checked in each src/naive_*.cu run_* function).  The first entry
+// whose algorithm-name fragment occurs in the algorithm's name applies.
+struct key_limit {
+    const char* key;
+    const char* alg;  // "" = any algorithm
+    long max;
+    const char* message;
+};
+const key_limit kLimits[] = {
+    {"warps_per_thread_block", "multirow_both_multimat", 32, "Cuda block too large"},
+    {"warps_per_thread_block", "", 32, "Too many warps per thread block"},
+    {"shifts_per_thread_block", "", 32, "Too many shifts per thread block"},
+    {"block_size", "", 1024, "CUDA block too large"},
+    {"right_matrices_per_block", "", 8, "Invalid number of right matrices per block"},
+    {"right_matrices_per_thread", "multimat_both", 4, "Too many right matrices per thread"},
+    {"right_matrices_per_thread", "multirow", 4, "Invalid number of right matrices per thread"},
+    {"right_matrices_per_thread", "", 8, "Invalid number of right matrices per thread"},
+    {"left_matrices_per_thread", "", 4, "Too many left matrices per thread"},
+    {"shifts_per_thread_right_matrix", "", 4, "Too many shifts per thread in each right matrix"},
+    {"shifts_per_thread", "", 8, "Too many shifts per thread"},
+    {"left_rows_per_iteration", "", 4, "Too many left rows per iteration"},
+    {"right_rows_per_thread", "multimat_right", 4, "Invalid number of right rows per thread"},
+    {"right_rows_per_thread", "", 8, "Too many right rows per thread"},
+};
+
 // Launch-time checks of the tuning values (errors of the run step, exit code 3).
-void check_tuning(const resolved_args& r) {
+void check_tuning(const std::string& alg, const resolved_args& r) {
     for (const auto& kv : r.ints) {
         const std::string& k = kv.first;
         if (k == "strided_load" || k == "column_group_per_block" || k == "tile" || k == "border") continue;
         if (kv.second <= 0) throw std::runtime_error("Invalid value " + std::to_string(kv.second) + " for " + k);
-        if (k == "warps_per_thread_block" && kv.second > 32)
-            throw std::runtime_error("Too many warps per thread block: " + std::to_string(kv.second) + " (max 32)");
+        for (const auto& l : kLimits) {
+            if (k != l.key || alg.find(l.alg) == std::string::npos) continue;
+            if (kv.second > l.max)
+                throw std::runtime_error(std::string(l.message) + ": " + std::to_string(kv.second) + " (max " +
+                                         std::to_string(l.max) + ")");
+            break;
+        }
     }
+}
+
+// The direct kernel's tile and schedule for a definition-based algorithm: each tuning key of
+// the reference's kernel selects the corresponding dimension of the tile (xc_options in
+// xcorr.h); dimensions the algorithm has no key for are 1, as in the reference's kernel
+// (one shift, one matrix, one row per step).  A 32-lane warp is half a 64-lane wave;
+// nai_orig_* keeps the reference's 16 x 16 thread block (4 waves).
+xc_options direct_options(const std::string& alg, const resolved_args& r) {
+    auto get = [&](const char* k) -> long { return r.ints.count(k) ? r.ints.at(k) : 0; };
+    xc_options o{};
+    o.precision = XC_PREC_FP32;
+    o.shifts_per_thread = o.right_mats_per_thread = o.left_mats_per_thread = o.rows_per_step = 1;
+    o.waves_per_block = 4;
+    if (long v = get("warps_per_thread_block")) o.waves_per_block = (uint32_t)((v + 1) / 2);
+    if (long v = get("shifts_per_thread_block")) o.waves_per_block = (uint32_t)((v + 1) / 2);
+    if (long v = get("block_size")) o.waves_per_block = (uint32_t)((v + 63) / 64);
+    for (const char* k : {"shifts_per_thread", "shifts_per_thread_right_matrix", "right_rows_per_thread"})
+        if (long v = get(k)) o.shifts_per_thread = (uint32_t)v;
+    for (const char* k : {"right_matrices_per_thread", "right_matrices_per_block"})
+        if (long v = get(k)) o.right_mats_per_thread = (uint32_t)v;
+    if (long v = get("left_matrices_per_thread")) o.left_mats_per_thread = (uint32_t)v;
+    if (long v = get("left_rows_per_iteration")) o.rows_per_step = (uint32_t)v;
+    // work distribution: at most rows_per_thread (rows_per_warp) overlap rows per worker
+    long rpt = get("rows_per_thread");
+    if (!rpt) rpt = get("rows_per_warp");
+    const std::string dist = r.texts.count("distribution_type") ? r.texts.at("distribution_type") : "none";
+    if (rpt > 0 && dist == "triangle") o.rows_per_block = (uint32_t)rpt;
+    (void)alg;
+    return o;
 }
 
 // ---------------------------------------------------------------- run options
@@ -528,19 +592,23 @@ public:
 
     void run() {
         switch (spec_.meth) {
-        case method::cpu: cpu_family<T, T>(spec_.family, left_, right_, out_); return;
+        case method::cpu: cpu_family<T, T>(spec_.family, left_, right_, out_, 1); return;
         case method::fft: run_fft(); return;
         default: break;
         }
         xc_options o{};
+        if (spec_.meth == method::direct) {
+            o = direct_options(spec_.name, args_);
+            // rectangle distribution: the same number of workers, ceil(rows / rows_per_thread),
+            // for every output row
+            long rpt = args_.ints.count("rows_per_thread") ? args_.ints.at("rows_per_thread") : 0;
+            if (args_.ints.count("rows_per_warp")) rpt = args_.ints.at("rows_per_warp");
+            const bool rect = args_.texts.count("distribution_type") && args_.texts.at("distribution_type") == "rectangle";
+            if (rpt > 0 && rect) o.split = (uint32_t)((left_.rows + rpt - 1) / rpt);
+        }
         o.precision = spec_.meth == method::autosel ? XC_PREC_AUTO : spec_.precision;
         if (args_.ints.count("tile")) o.tile = (uint32_t)args_.ints.at("tile");
         if (args_.ints.count("border")) o.border = (uint32_t)args_.ints.at("border");
-        // work distribution: at most rows_per_thread (rows_per_warp) overlap rows per worker
-        long rpt = 0;
-        if (args_.ints.count("rows_per_thread")) rpt = args_.ints.at("rows_per_thread");
-        if (args_.ints.count("rows_per_warp")) rpt = args_.ints.at("rows_per_warp");
-        if (rpt > 0) o.split = (uint32_t)((left_.rows + rpt - 1) / rpt);
         auto body = [&] { xc_ok(call(o)); };
         kernel_ = adaptive(measure_alg() ? min_ns_ : 0, true, body);
     }
@@ -562,7 +630,7 @@ public:
         const matrix_array<T> res = is_fft() ? normalize_fft(out_) : out_;
         if (!valid_path.empty()) return validate_result(res, load_csv<T>(valid_path));
         matrix_array<T> valid(2 * H() - 1, 2 * W() - 1, k_);
-        cpu_family<double, T>(spec_.family, left_, right_, valid);
+        cpu_family<T, T>(spec_.family, left_, right_, valid, 0);
         return validate_result(res, valid);
     }
 
@@ -807,7 +875,7 @@ int run_measurement(const run_options& ro, const alg_spec& spec) {
             // Tuning values are checked before any device work, so a bad value fails the
             // same way with or without a GPU.
             if (int rc = step("Allocating", "allocation", 1, [&] {
-                    check_tuning(args);
+                    check_tuning(spec.name, args);
                     alg.prepare();
                 }))
                 return rc;

@@ -111,7 +111,7 @@ xc_status xc_cross_corr_f64(xc_family family, const double* left, const double* 
         else if (o.precision == XC_PREC_FFT64)
             xc::run_fft_f64(a, left, right, out, true, (hipStream_t)stream);
         else
-            xc::launch_direct_f64(a, left, right, out, o.split, (hipStream_t)stream);
+            xc::launch_direct_f64(a, left, right, out, o, (hipStream_t)stream);
     });
 }
 
@@ -257,9 +257,9 @@ xc_status xc_fft_cross_corr_f64(xc_family family, const double* left, const doub
     });
 }
 
-// run_ccn_shuffle (src/naive_shuffle.cu:452-484): one_to_one; warps_per_thread_block in
-// [1, 32] as the reference checks it (:461-463).  The wave64 kernel has a fixed block shape,
-// so the value is validated and recorded but does not change the launch.
+// run_ccn_shuffle (src/naive_shuffle.cu:452-484): one_to_one, one shift per thread;
+// warps_per_thread_block in [1, 32] as the reference checks it (:461-463), two 32-lane warps
+// per 64-lane wave of the block.
 xc_status xc_run_ccn_shuffle_f32(const float* left, const float* right, float* out, uint32_t cols, uint32_t rows,
                                  uint32_t search_cols, uint32_t search_rows, uint32_t warps_per_thread_block,
                                  xc_stream_t stream) {
@@ -271,13 +271,18 @@ xc_status xc_run_ccn_shuffle_f32(const float* left, const float* right, float* o
             xc::fail(XC_ERR_INVALID_ARGUMENT, "search size must be the full (2*rows-1) x (2*cols-1)");
         check_family(XC_ONE_TO_ONE, rows, cols, 1, 1, left, right, out);
         xc::family_args a{XC_ONE_TO_ONE, rows, cols, 1, 1, 1};
-        xc::launch_direct_f32(a, left, right, out, 0, (hipStream_t)stream);
+        xc_options o{};
+        o.shifts_per_thread = o.right_mats_per_thread = o.left_mats_per_thread = o.rows_per_step = 1;
+        o.waves_per_block = (warps_per_thread_block + 1) / 2;
+        xc::launch_direct_f32(a, left, right, out, o, (hipStream_t)stream);
     });
 }
 
 // run_ccn_shuffle_one_to_many_multirow_both_multimat_right
 // (src/naive_shuffle_one_to_many_multirow_both_multimat_right.cu:776-833): the reference's
-// argument checks (:789-818) are kept; the register tile of this library is its own.
+// argument checks with the limits of its release build (:789-818, CMakeLists.txt:41-43);
+// the values select the direct kernel's tile: shifts_per_thread_right_matrix consecutive dy,
+// right_matrices_per_thread matrices and left_rows_per_iteration right rows per step.
 xc_status xc_run_ccn_shuffle_one_to_many_multirow_both_multimat_right_f32(
     const float* left, const float* right, float* out, uint32_t cols, uint32_t rows, uint32_t search_cols,
     uint32_t search_rows, uint32_t num_right_matrices, uint32_t warps_per_thread_block,
@@ -285,22 +290,28 @@ xc_status xc_run_ccn_shuffle_one_to_many_multirow_both_multimat_right_f32(
     xc_stream_t stream) {
     return guarded([&] {
         if (warps_per_thread_block == 0 || warps_per_thread_block > 32)
-            xc::fail(XC_ERR_INVALID_ARGUMENT, "Too many warps per thread block: " +
-                                                  std::to_string(warps_per_thread_block) + " (max 32)");
-        if (shifts_per_thread_right_matrix == 0 || shifts_per_thread_right_matrix > 32)
-            xc::fail(XC_ERR_INVALID_ARGUMENT, "Invalid number of shifts per thread: " +
-                                                  std::to_string(shifts_per_thread_right_matrix));
-        if (right_matrices_per_thread == 0 || right_matrices_per_thread > 32)
+            xc::fail(XC_ERR_INVALID_ARGUMENT,
+                     "Cuda block too large: " + std::to_string(warps_per_thread_block) + " (max 32)");
+        if (shifts_per_thread_right_matrix == 0 || shifts_per_thread_right_matrix > 4)
+            xc::fail(XC_ERR_INVALID_ARGUMENT, "Too many shifts per thread in each right matrix: " +
+                                                  std::to_string(shifts_per_thread_right_matrix) + " (max 4)");
+        if (right_matrices_per_thread == 0 || right_matrices_per_thread > 4)
             xc::fail(XC_ERR_INVALID_ARGUMENT, "Invalid number of right matrices per thread: " +
-                                                  std::to_string(right_matrices_per_thread));
-        if (left_rows_per_iteration == 0 || left_rows_per_iteration > 32)
-            xc::fail(XC_ERR_INVALID_ARGUMENT, "Invalid number of left rows per iteration: " +
-                                                  std::to_string(left_rows_per_iteration));
+                                                  std::to_string(right_matrices_per_thread) + " [1-4]");
+        if (left_rows_per_iteration == 0 || left_rows_per_iteration > 4)
+            xc::fail(XC_ERR_INVALID_ARGUMENT, "Too many left rows per iteration: " +
+                                                  std::to_string(left_rows_per_iteration) + " (max 4)");
         if (search_cols != 2 * cols - 1 || search_rows != 2 * rows - 1)
             xc::fail(XC_ERR_INVALID_ARGUMENT, "search size must be the full (2*rows-1) x (2*cols-1)");
         check_family(XC_ONE_TO_MANY, rows, cols, 1, num_right_matrices, left, right, out);
         xc::family_args a{XC_ONE_TO_MANY, rows, cols, 1, num_right_matrices, num_right_matrices};
-        xc::launch_direct_f32(a, left, right, out, 0, (hipStream_t)stream);
+        xc_options o{};
+        o.shifts_per_thread = shifts_per_thread_right_matrix;
+        o.right_mats_per_thread = right_matrices_per_thread;
+        o.left_mats_per_thread = 1;
+        o.rows_per_step = left_rows_per_iteration;
+        o.waves_per_block = (warps_per_thread_block + 1) / 2;
+        xc::launch_direct_f32(a, left, right, out, o, (hipStream_t)stream);
     });
 }
 

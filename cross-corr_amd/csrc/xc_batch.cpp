@@ -521,7 +521,7 @@ void run_batch_f32(const xc_pair* pairs, uint32_t num_pairs, const float* L, con
         // Exact FP32 products: one direct-kernel launch per pair.
         for (uint32_t i = 0; i < num_pairs; ++i) {
             family_args a{XC_ONE_TO_ONE, pairs[i].rows, pairs[i].cols, 1, 1, 1};
-            launch_direct_f32(a, L + pairs[i].left_off, R + pairs[i].right_off, O + pairs[i].out_off, opt.split, s);
+            launch_direct_f32(a, L + pairs[i].left_off, R + pairs[i].right_off, O + pairs[i].out_off, opt, s);
         }
         return;
     }

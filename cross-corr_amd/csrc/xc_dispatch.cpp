@@ -77,7 +77,7 @@ void run_family_f32(const family_args& a, const float* L, const float* R, float*
         run_family_f32(a, L, R, O, auto_options_f32(a, L, R, O, s), s);
         return;
     case XC_PREC_FP32:
-        launch_direct_f32(a, L, R, O, opt.split, s);
+        launch_direct_f32(a, L, R, O, opt, s);
         return;
     case XC_PREC_BF16X3:
     case XC_PREC_FP16X3:

@@ -99,9 +99,11 @@ struct family_args {
 };
 
 // Kernel launchers (defined in the .hip files).
-void launch_direct_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t split,
+// Direct kernel: opt's split, rows_per_block and register-tile fields (xcorr.h) pick the
+// schedule; its precision and tile fields are not read.
+void launch_direct_f32(const family_args& a, const float* L, const float* R, float* O, const xc_options& opt,
                        hipStream_t s);
-void launch_direct_f64(const family_args& a, const double* L, const double* R, double* O, uint32_t split,
+void launch_direct_f64(const family_args& a, const double* L, const double* R, double* O, const xc_options& opt,
                        hipStream_t s);
 void launch_zero_f32(float* p, uint64_t count, hipStream_t s);
 void launch_zero_f64(double* p, uint64_t count, hipStream_t s);

@@ -91,7 +91,21 @@ typedef struct xc_options {
                              FFT64 tile 1: at most this many pairs per launch of the column
                              and inverse-row kernels (less scratch memory); 0 = auto */
     uint32_t border;      /* BF16X3: overlap-area threshold for exact FP32 recompute, 0 = default */
-    uint32_t reserved[4];
+    /* Register tile and schedule of the direct kernel (XC_PREC_FP32, and double data's
+     * direct method); 0 = the library's default for the family and size.  Each value is
+     * rounded down to a power of two and the tile to the largest compiled one not above the
+     * request in any dimension (xc_last_launch names the tile used).  They correspond to the
+     * reference's tuning arguments:                                                       */
+    uint32_t shifts_per_thread;     /* consecutive dy per thread, 1..16 (shifts_per_thread,
+                                       right_rows_per_thread, shifts_per_thread_right_matrix) */
+    uint32_t right_mats_per_thread; /* right matrices per thread, 1..8 (right_matrices_per_thread) */
+    uint32_t left_mats_per_thread;  /* left matrices per thread, 1..4, n_to_m (left_matrices_per_thread) */
+    uint32_t rows_per_step;         /* right rows per LDS step, 1..4 (left_rows_per_iteration) */
+    uint32_t waves_per_block;       /* 64-lane waves per block, 1..8 (warps_per_thread_block / 2) */
+    uint32_t rows_per_block;        /* triangle work distribution: at most this many overlap rows
+                                       per block, blocks add atomically (max_rows_per_thread);
+                                       exclusive with split, which is the rectangle distribution */
+    uint32_t reserved[2];
 } xc_options;
 
 /* ---- family entry points (the drop-in for the L5 wrappers' run_impl) ----------------
@@ -202,9 +216,13 @@ xc_status xc_fft_cross_corr_f64(xc_family family, const double* left, const doub
                                 xc_stream_t stream);
 
 /* ---- reference run_* launchers with their tuning arguments --------------------------
- * Same argument meaning as src/kernels.cuh; tuning arguments are validated with the
- * reference's limits (e.g. warps_per_thread_block <= 32, src/naive_shuffle.cu:461-463)
- * and mapped onto this library's kernels; matrix_size/search_size are {cols, rows}. */
+ * Same argument meaning as src/kernels.cuh:65-73 and :223-236; tuning arguments are
+ * validated with the limits of the reference's release build (warps_per_thread_block <= 32,
+ * src/naive_shuffle.cu:461-463; shifts, right matrices and left rows per iteration <= 4,
+ * src/naive_shuffle_one_to_many_multirow_both_multimat_right.cu:789-818) and select the
+ * direct kernel's tile (see xc_options); matrix_size/search_size are {cols, rows}.  The other
+ * run_* launchers of kernels.cuh are header templates over xc_cross_corr_* in
+ * xcorr_kernels.hpp. */
 xc_status xc_run_ccn_shuffle_f32(const float* left, const float* right, float* out,
                                  uint32_t cols, uint32_t rows, uint32_t search_cols, uint32_t search_rows,
                                  uint32_t warps_per_thread_block, xc_stream_t stream);

@@ -114,6 +114,34 @@ def test_validate_command_csv_and_header():
                                      "0.000000e+00,0.000000e+00,0.000000e+00"]
 
 
+def _write_csv(path, a):
+    a = np.asarray(a, dtype=np.float64)
+    with open(path, "w") as f:
+        f.write(f"# {a.shape[0]},{a.shape[1]}\n")
+        for row in a:
+            f.write(",".join(repr(float(v)) for v in row) + "\n")
+
+
+@pytest.mark.parametrize("result,valid", [
+    ([[1.0, 2.0], [0.0, 1e-6]], [[1.0, 1.0], [0.0, 0.0]]),          # max 0.5, mean 0.125
+    ([[2e-5, 3.0], [4.0, -1.0]], [[1e-5, 3.0], [5.0, -1.5]]),
+    ([[1.5, -2.0, 7.0]], [[1.0, 2.0, 7.5]]),
+])
+def test_validate_command_matches_the_oracle_metric(tmp_path, result, valid):
+    # The CLI's validate_result (src/validate.hpp:134-188) against the oracle's restatement,
+    # which the hand-computed cases of tests/test_oracle.py pin.
+    from xc_testutil import validate_result
+    rp, vp = tmp_path / "result.csv", tmp_path / "valid.csv"
+    _write_csv(rp, result)
+    _write_csv(vp, valid)
+    r = cross("validate", "--csv", vp, rp, check=0)
+    got = [float(v) for v in r.stdout.strip().split(",")]
+    want = validate_result(np.ravel(result), np.ravel(valid))[:3]
+    assert np.allclose(got, want, rtol=1e-6, atol=0), (got, want)
+    if result[0][:2] == [1.0, 2.0]:
+        assert got[0] == 0.5 and got[1] == 0.125
+
+
 def test_timing_csv_compute_mode_and_append(tmp_path):
     t = tmp_path / "times.csv"
     args = ["-b", "Compute", "-m", "0.001", "-t", t, "--no_progress", "cpu_one_to_one",
@@ -167,6 +195,33 @@ def test_tuning_limits(tmp_path):
     r = cross("run", "--args_path", args, "nai_shuffle_work_distribution_one_to_one", DATA / "ina_10_10_1_1.csv",
               DATA / "ina_10_10_1_2.csv")
     assert r.returncode == 2 and "distribution" in r.stderr
+
+
+@pytest.mark.parametrize("alg,key,value,needle", [
+    ("nai_shuffle_multirow_both_one_to_one", "shifts_per_thread", 9, "Too many shifts per thread: 9 (max 8)"),
+    ("nai_shuffle_multirow_both_one_to_one", "left_rows_per_iteration", 5, "left rows per iteration: 5 (max 4)"),
+    ("nai_shuffle_multimat_right_one_to_many", "right_matrices_per_thread", 9, "right matrices per thread: 9"),
+    ("nai_shuffle_multirow_both_multimat_right_one_to_many", "shifts_per_thread_right_matrix", 5,
+     "shifts per thread in each right matrix: 5 (max 4)"),
+    ("nai_shuffle_multirow_both_multimat_right_one_to_many", "warps_per_thread_block", 33, "Cuda block too large"),
+    ("nai_shuffle_multirow_right_multimat_right_one_to_many", "right_rows_per_thread", 5,
+     "right rows per thread: 5 (max 4)"),
+    ("nai_shuffle_multirow_right_one_to_one", "right_rows_per_thread", 9, "right rows per thread: 9 (max 8)"),
+    ("nai_shuffle_multimat_both_work_distribution_n_to_m", "left_matrices_per_thread", 5,
+     "left matrices per thread: 5 (max 4)"),
+    ("nai_block_per_shift_one_to_one", "block_size", 2048, "block too large"),
+    ("nai_shuffle_multirow_both_one_to_one", "shifts_per_thread", 0, "Invalid value 0"),
+])
+def test_reference_release_limits(tmp_path, alg, key, value, needle):
+    # The limits of the reference's release build (CMakeLists.txt:16-45) refuse the value at
+    # the allocation step, before any device work: exit code 3.
+    args = tmp_path / "args.json"
+    args.write_text(json.dumps({key: value}))
+    fam = alg.rsplit("_", 2)
+    right = "ina_10_10_2_1.csv" if "one_to_many" in alg or "n_to_m" in alg else "ina_10_10_1_2.csv"
+    left = "ina_10_10_2_1.csv" if "n_to_m" in alg and "one_to_many" not in alg else "ina_10_10_1_1.csv"
+    r = cross("run", "--args_path", args, alg, DATA / left, DATA / right)
+    assert r.returncode == 3 and needle in r.stderr, (r.returncode, r.stderr, fam)
 
 
 def test_runner_invocation_shape_cpu(tmp_path):

@@ -101,6 +101,62 @@ def test_direct_split_reduction(split):
     check("one_to_many", left, right, run("one_to_many", left, right, precision="fp32", split=split), 1e-6)
 
 
+# Register tiles of the direct kernel: every request of the grid below is rounded to a
+# compiled tile (named in the launch descriptor) and must give the oracle's results, with
+# partial groups of left and right matrices, outputs wider than one wave and every family.
+TILE_SHAPES = [("n_to_m", 19, 37, 3, 5), ("one_to_many", 33, 70, 1, 7), ("n_to_mn", 20, 9, 2, 6),
+               ("one_to_one", 11, 13, 1, 1)]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("ty", [1, 2, 4, 8, 16])
+def test_direct_tile_catalog(ty, dtype):
+    tol = 1e-6 if dtype == np.float32 else 1e-12
+    used = set()
+    for rm in (1, 2, 4, 8):
+        for lm in (1, 2, 4):
+            for jb in (1, 2, 4):
+                for family, rows, cols, n, m in TILE_SHAPES:
+                    left = seeded((n, rows, cols), "normal", 500 + rows, dtype)
+                    right = seeded((m, rows, cols), "normal", 600 + cols, dtype)
+                    print(f"tile request ty{ty} rm{rm} lm{lm} jb{jb} {family} {dtype.__name__}", flush=True)
+                    got = run(family, left, right, precision="fp32", shifts_per_thread=ty, right_mats_per_thread=rm,
+                              left_mats_per_thread=lm, rows_per_step=jb)
+                    used.add(xcorr().last_launch()["kernel"])
+                    check(family, left, right, got, tol)
+    assert len(used) >= 3, used
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+def test_direct_waves_per_block(waves):
+    for family, rows, cols, n, m in TILE_SHAPES + [("one_to_many", 70, 300, 1, 3)]:
+        left = seeded((n, rows, cols), "u11", 700 + rows, np.float32)
+        right = seeded((m, rows, cols), "u11", 800 + cols, np.float32)
+        print(f"waves {waves} {family} {rows}x{cols}", flush=True)
+        got = run(family, left, right, precision="fp32", waves_per_block=waves)
+        assert xcorr().last_launch()["block"][0] == 64 * waves
+        check(family, left, right, got, 1e-6)
+
+
+@pytest.mark.parametrize("rows_per_block", [1, 3, 8, 40])
+def test_direct_triangle_distribution(rows_per_block):
+    # blocks take at most rows_per_block overlap rows (more blocks where the overlap is larger)
+    for dtype, tol in ((np.float32, 1e-6), (np.float64, 1e-12)):
+        left = seeded((2, 45, 30), "u01", 17, dtype)
+        right = seeded((3, 45, 30), "u01", 18, dtype)
+        for jb in (1, 4):
+            got = run("n_to_m", left, right, precision="fp32", rows_per_block=rows_per_block, rows_per_step=jb)
+            assert xcorr().last_launch()["split"] == (45 + max(rows_per_block, 1) - 1) // max(rows_per_block, 1) or jb > 1
+            check("n_to_m", left, right, got, tol)
+
+
+def test_direct_split_and_triangle_are_exclusive():
+    x = xcorr()
+    left = dev(seeded((1, 8, 8), "u01", 1, np.float32))
+    with pytest.raises(x.XcError, match="not both"):
+        x.cross_corr("one_to_one", left, left, precision="fp32", split=2, rows_per_block=3)
+
+
 def test_c1_config_one_to_one_128():
     left = seeded((1, 128, 128), "u01", 1, np.float32)
     right = seeded((1, 128, 128), "u01", 2, np.float32)
