@@ -81,6 +81,7 @@ _st = ctypes.c_int
 SIGNATURES = {
     "xc_cross_corr_f32": (_st, [_i32, _vp, _vp, _vp, _u32, _u32, _u32, _u32, ctypes.POINTER(Options), _vp]),
     "xc_cross_corr_f64": (_st, [_i32, _vp, _vp, _vp, _u32, _u32, _u32, _u32, ctypes.POINTER(Options), _vp]),
+    "xc_cross_corr_noalloc_f32": (_st, [_i32, _vp, _vp, _vp, _u32, _u32, _u32, _u32, _vp]),
     "xc_num_results": (ctypes.c_int64, [_i32, _u32, _u32]),
     "xc_batch_f32": (_st, [ctypes.POINTER(Pair), _u32, _vp, _vp, _vp, ctypes.POINTER(Options), _vp]),
     "xc_batch_plan_create_f32": (_st, [ctypes.POINTER(Pair), _u32, ctypes.POINTER(Options),

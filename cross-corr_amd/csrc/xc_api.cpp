@@ -96,6 +96,15 @@ xc_status xc_cross_corr_f32(xc_family family, const float* left, const float* ri
     });
 }
 
+xc_status xc_cross_corr_noalloc_f32(xc_family family, const float* left, const float* right, float* out,
+                                    uint32_t rows, uint32_t cols, uint32_t n, uint32_t m, xc_stream_t stream) {
+    return guarded([&] {
+        check_family(family, rows, cols, n, m, left, right, out);
+        xc::family_args a{family, rows, cols, n, m, (uint32_t)xc::num_results(family, n, m)};
+        xc::run_fftk_noalloc_f32(a, left, right, out, (hipStream_t)stream);
+    });
+}
+
 xc_status xc_cross_corr_f64(xc_family family, const double* left, const double* right, double* out,
                             uint32_t rows, uint32_t cols, uint32_t n, uint32_t m, const xc_options* opt,
                             xc_stream_t stream) {

@@ -1071,75 +1071,80 @@ int ilog2_ceil(uint32_t x) {
     return e;
 }
 
-// tw[j] = exp(-2 pi i j / 4096), one table per device, kept for the process.
-const double2* twiddles() {
-    static std::mutex mu;
-    static std::vector<double2*> tables;
-    int dev = 0;
-    XC_HIP(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lock(mu);
-    if ((int)tables.size() <= dev) tables.resize(dev + 1, nullptr);
-    if (!tables[dev]) {
-        std::vector<double2> h(kSlab);
-        const double two_pi = 6.283185307179586476925286766559;
-        for (int j = 0; j < kSlab; ++j) {
-            // exact at the eighths of the circle, where cos/sin of the rounded argument are not
-            const int oct = j % (kSlab / 8);
-            if (oct == 0) {
-                static const double c8[8][2] = {{1, 0},
-                                                {0.70710678118654752440, -0.70710678118654752440},
-                                                {0, -1},
-                                                {-0.70710678118654752440, -0.70710678118654752440},
-                                                {-1, 0},
-                                                {-0.70710678118654752440, 0.70710678118654752440},
-                                                {0, 1},
-                                                {0.70710678118654752440, 0.70710678118654752440}};
-                h[j] = make_double2(c8[j / (kSlab / 8)][0], c8[j / (kSlab / 8)][1]);
-            } else {
-                const double a = two_pi * (double)j / (double)kSlab;
-                h[j] = make_double2(std::cos(a), -std::sin(a));
-            }
-        }
-        double2* d = nullptr;
-        if (hipMalloc(&d, sizeof(double2) * kSlab) != hipSuccess) {
-            (void)hipGetLastError();
-            fail(XC_ERR_NO_MEMORY, "twiddle table allocation failed");
-        }
-        XC_HIP(hipMemcpy(d, h.data(), sizeof(double2) * kSlab, hipMemcpyHostToDevice));
-        tables[dev] = d;
-    }
-    return tables[dev];
-}
+// Twiddle tables in device memory of the code object (no allocation): tw[j] =
+// exp(-2 pi i j / 4096) and tw3[j] = exp(-2 pi i j / 12288).  The host computes them and
+// copies them in once per device.
+__device__ double2 g_tw[kSlab];
+__device__ double2 g_tw3[kTw3];
 
-const double2* twiddles3() {
-    static std::mutex mu;
-    static std::vector<double2*> tables;
-    int dev = 0;
-    XC_HIP(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lock(mu);
-    if ((int)tables.size() <= dev) tables.resize(dev + 1, nullptr);
-    if (!tables[dev]) {
-        std::vector<double2> h(kTw3);
-        const double two_pi = 6.283185307179586476925286766559;
-        for (int j = 0; j < kTw3; ++j) {
-            const double a = two_pi * (double)j / (double)kTw3;
+std::vector<double2> twiddle_values() {
+    std::vector<double2> h(kSlab);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int j = 0; j < kSlab; ++j) {
+        // exact at the eighths of the circle, where cos/sin of the rounded argument are not
+        const int oct = j % (kSlab / 8);
+        if (oct == 0) {
+            static const double c8[8][2] = {{1, 0},
+                                            {0.70710678118654752440, -0.70710678118654752440},
+                                            {0, -1},
+                                            {-0.70710678118654752440, -0.70710678118654752440},
+                                            {-1, 0},
+                                            {-0.70710678118654752440, 0.70710678118654752440},
+                                            {0, 1},
+                                            {0.70710678118654752440, 0.70710678118654752440}};
+            h[j] = make_double2(c8[j / (kSlab / 8)][0], c8[j / (kSlab / 8)][1]);
+        } else {
+            const double a = two_pi * (double)j / (double)kSlab;
             h[j] = make_double2(std::cos(a), -std::sin(a));
         }
-        // exact at the twelfths of the circle (the radix-3 and radix-4 points)
-        const double s3 = 0.86602540378443864676;
-        const double c12[12][2] = {{1, 0},   {s3, -0.5}, {0.5, -s3}, {0, -1},  {-0.5, -s3}, {-s3, -0.5},
-                                   {-1, 0},  {-s3, 0.5}, {-0.5, s3}, {0, 1},   {0.5, s3},   {s3, 0.5}};
-        for (int q = 0; q < 12; ++q) h[q * (kTw3 / 12)] = make_double2(c12[q][0], c12[q][1]);
-        double2* d = nullptr;
-        if (hipMalloc(&d, sizeof(double2) * kTw3) != hipSuccess) {
-            (void)hipGetLastError();
-            fail(XC_ERR_NO_MEMORY, "twiddle table allocation failed");
-        }
-        XC_HIP(hipMemcpy(d, h.data(), sizeof(double2) * kTw3, hipMemcpyHostToDevice));
-        tables[dev] = d;
     }
-    return tables[dev];
+    return h;
 }
+
+std::vector<double2> twiddle3_values() {
+    std::vector<double2> h(kTw3);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int j = 0; j < kTw3; ++j) {
+        const double a = two_pi * (double)j / (double)kTw3;
+        h[j] = make_double2(std::cos(a), -std::sin(a));
+    }
+    // exact at the twelfths of the circle (the radix-3 and radix-4 points)
+    const double s3 = 0.86602540378443864676;
+    const double c12[12][2] = {{1, 0},   {s3, -0.5}, {0.5, -s3}, {0, -1},  {-0.5, -s3}, {-s3, -0.5},
+                               {-1, 0},  {-s3, 0.5}, {-0.5, s3}, {0, 1},   {0.5, s3},   {s3, 0.5}};
+    for (int q = 0; q < 12; ++q) h[q * (kTw3 / 12)] = make_double2(c12[q][0], c12[q][1]);
+    return h;
+}
+
+struct device_tables {
+    bool ready = false;
+    double2* tw = nullptr;
+    double2* tw3 = nullptr;
+};
+
+// The current device's twiddle tables: on its first use the symbols' addresses are looked up
+// and the values copied in (synchronously, before any kernel reads them).
+const device_tables& tables_of_device() {
+    static std::mutex mu;
+    static std::vector<device_tables> per_dev;
+    int dev = 0;
+    XC_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1);
+    device_tables& t = per_dev[dev];
+    if (!t.ready) {
+        XC_HIP(hipGetSymbolAddress(reinterpret_cast<void**>(&t.tw), HIP_SYMBOL(g_tw)));
+        XC_HIP(hipGetSymbolAddress(reinterpret_cast<void**>(&t.tw3), HIP_SYMBOL(g_tw3)));
+        const std::vector<double2> h = twiddle_values(), h3 = twiddle3_values();
+        XC_HIP(hipMemcpy(t.tw, h.data(), sizeof(double2) * kSlab, hipMemcpyHostToDevice));
+        XC_HIP(hipMemcpy(t.tw3, h3.data(), sizeof(double2) * kTw3, hipMemcpyHostToDevice));
+        t.ready = true;
+    }
+    return t;
+}
+
+const double2* twiddles() { return tables_of_device().tw; }
+const double2* twiddles3() { return tables_of_device().tw3; }
 
 struct slab_shape {
     int logS;      // log2 of the slab (values per block)
@@ -1378,7 +1383,154 @@ void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, uint32_t
     launch_job<Tio>(j, L, R, O, ws, s, true);
 }
 
+// ---- allocation-free entry point ---------------------------------------------------------
+// The row spectra S, the left spectra Lhat and the column-pass results Z of
+// xc_cross_corr_noalloc_f32 live in this array of the code object instead of allocated
+// workspaces.  Every device that runs the library holds one copy for the process; calls on
+// one device share it, so calls that overlap on one device race (each call overwrites what it
+// reads before reading it, so nothing needs clearing between calls).
+constexpr size_t kScratchBytes = size_t(256) << 20;
+__device__ __attribute__((aligned(256))) char scratch[kScratchBytes];
+
+// The current device's address of `scratch`, looked up on the first call on that device.
+char* scratch_of_device() {
+    static std::mutex mu;
+    static std::vector<char*> ptrs;
+    int dev = 0;
+    XC_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)ptrs.size() <= dev) ptrs.resize(dev + 1, nullptr);
+    if (!ptrs[dev]) {
+        char* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            fail(XC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        ptrs[dev] = p;
+    }
+    return ptrs[dev];
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// One launch sequence of the allocation-free path: a regular family over a slice of the
+// call's matrices (element offsets of its first left, right and result).
+struct noalloc_part {
+    int family;
+    uint32_t n, m, pairs;  // lefts, rights, results of the part
+    size_t loff, roff, ooff;
+    uint32_t chunk;        // pairs per launch of the column / inverse-row kernels
+};
+
+// Scratch bytes of a part with nl left and nr right spectra and `chunk` pairs of Z.
+size_t part_bytes(const fftk_geom& g, uint32_t nl, uint32_t nr, uint32_t chunk) {
+    return align256(g.s_bytes * nr) + align256(g.s_bytes * nl) + align256(g.col_bytes * nl) +
+           align256(g.z_bytes * chunk);
+}
+
+// Splits a call into parts whose spectra fit the scratch array: the whole call if it fits,
+// else groups of right matrices (one_to_many, n_to_mn: whole groups of n), of left matrices
+// against every right (n_to_m), or of right matrices per left (n_to_m when the rights alone do
+// not fit).  Each part's results are contiguous in the output.
+std::vector<noalloc_part> plan_noalloc(const family_args& a, const fftk_geom& g) {
+    const size_t B = kScratchBytes, hw = (size_t)a.rows * a.cols;
+    const size_t out_elems = (size_t)(2 * a.rows - 1) * (2 * a.cols - 1);
+    const size_t one_pair = part_bytes(g, 1, 1, 1);
+    if (one_pair > B)
+        fail(XC_ERR_UNSUPPORTED, "allocation-free entry point: one pair of " + std::to_string(a.rows) + " x " +
+                                     std::to_string(a.cols) + " needs " + std::to_string(one_pair >> 20) +
+                                     " MiB of scratch, above its " + std::to_string(kScratchBytes >> 20) + " MiB");
+    std::vector<noalloc_part> parts;
+    // Right matrices of `rights` consecutive results, with `nl` lefts, chunk = all pairs.
+    auto fit_rights = [&](uint32_t nl, uint32_t unit, uint32_t most) -> uint32_t {
+        uint32_t g_best = 0;
+        for (uint32_t c = unit; c <= most; c += unit) {
+            if (part_bytes(g, nl, c, c) > B) break;
+            g_best = c;
+        }
+        return g_best;
+    };
+    switch (a.family) {
+    case XC_ONE_TO_ONE:
+        parts.push_back({XC_ONE_TO_ONE, 1, 1, 1, 0, 0, 0, 1});
+        break;
+    case XC_ONE_TO_MANY: {
+        const uint32_t gr = fit_rights(1, 1, a.m);
+        for (uint32_t r0 = 0; r0 < a.m; r0 += gr) {
+            const uint32_t c = std::min(gr, a.m - r0);
+            parts.push_back({XC_ONE_TO_MANY, 1, c, c, 0, r0 * hw, r0 * out_elems, c});
+        }
+        break;
+    }
+    case XC_N_TO_MN: {
+        const uint32_t gr = fit_rights(a.n, a.n, a.m);
+        if (gr == 0)
+            fail(XC_ERR_UNSUPPORTED, "allocation-free entry point: the spectra of " + std::to_string(a.n) +
+                                         " left matrices do not fit its scratch array");
+        for (uint32_t r0 = 0; r0 < a.m; r0 += gr) {
+            const uint32_t c = std::min(gr, a.m - r0);  // a multiple of n, as m is
+            parts.push_back({XC_N_TO_MN, a.n, c, c, 0, r0 * hw, r0 * out_elems, c});
+        }
+        break;
+    }
+    default: {  // n_to_m
+        uint32_t gl = 0;
+        for (uint32_t c = 1; c <= a.n; ++c) {
+            if (part_bytes(g, c, a.m, a.m) > B) break;
+            gl = c;
+        }
+        if (gl > 0) {
+            for (uint32_t l0 = 0; l0 < a.n; l0 += gl) {
+                const uint32_t c = std::min(gl, a.n - l0);
+                // as many pairs per launch as the rest of the array holds (at least m: gl fits them)
+                const size_t rest = B - part_bytes(g, c, a.m, 0);
+                uint32_t chunk = (uint32_t)std::min<size_t>((size_t)c * a.m, rest / g.z_bytes);
+                while (chunk > a.m && part_bytes(g, c, a.m, chunk) > B) --chunk;
+                parts.push_back({XC_N_TO_M, c, a.m, c * a.m, l0 * hw, 0, (size_t)l0 * a.m * out_elems, chunk});
+            }
+        } else {
+            const uint32_t gr = fit_rights(1, 1, a.m);
+            for (uint32_t l = 0; l < a.n; ++l)
+                for (uint32_t r0 = 0; r0 < a.m; r0 += gr) {
+                    const uint32_t c = std::min(gr, a.m - r0);
+                    parts.push_back({XC_ONE_TO_MANY, 1, c, c, l * hw, r0 * hw, ((size_t)l * a.m + r0) * out_elems, c});
+                }
+        }
+    }
+    }
+    return parts;
+}
+
 }  // namespace
+
+void run_fftk_noalloc_f32(const family_args& a, const float* L, const float* R, float* O, hipStream_t s) {
+    char* base = scratch_of_device();
+    const fftk_geom g = geom_of(a.rows, a.cols);
+    (void)tables_of_device();
+    for (const noalloc_part& p : plan_noalloc(a, g)) {
+        fftk_job j{};
+        j.H = a.rows;
+        j.W = a.cols;
+        j.family = p.family;
+        j.n = p.n;
+        j.m = p.m;
+        j.nl = p.family == XC_ONE_TO_ONE || p.family == XC_ONE_TO_MANY ? 1 : p.n;
+        j.nr = p.m;
+        j.pairs = p.pairs;
+        fftk_ws ws{};
+        char* q = base;
+        ws.sr = q;
+        q += align256(g.s_bytes * j.nr);
+        ws.sl = q;
+        q += align256(g.s_bytes * j.nl);
+        ws.lh = q;
+        q += align256(g.col_bytes * j.nl);
+        ws.z = q;
+        ws.zchunk = p.chunk;
+        if ((size_t)(q - base) + align256(g.z_bytes * p.chunk) > kScratchBytes)
+            fail(XC_ERR_INVALID_ARGUMENT, "allocation-free entry point: internal scratch plan out of bounds");
+        launch_job<float>(j, L + p.loff, R + p.roff, O + p.ooff, ws, s, false);
+    }
+}
 
 uint32_t fftk_col_len(uint32_t rows) {
     uint32_t p = 16;

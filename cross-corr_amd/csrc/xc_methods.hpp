@@ -93,6 +93,9 @@ void fftk_prepare();  // device tables the kernels read (call before stream capt
 void fftk_launch_f32(const fftk_job& j, const float* L, const float* R, float* O, const fftk_ws& ws, hipStream_t s);
 
 // FFT method on the library's own FP64 FFT kernels (xc_fftk.hip): rows, cols <= 2048.
+// The own FFT without any allocation: scratch memory is a fixed array of the code object
+// (rows, cols <= 2048; one pair's spectra must fit it).
+void run_fftk_noalloc_f32(const family_args& a, const float* L, const float* R, float* O, hipStream_t s);
 // max_chunk: pairs per Z chunk as fftk_job::max_chunk (0 = default).
 void run_fftk_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t max_chunk, hipStream_t s);
 void run_fftk_f64(const family_args& a, const double* L, const double* R, double* O, uint32_t max_chunk,

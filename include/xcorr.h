@@ -119,6 +119,21 @@ xc_status xc_cross_corr_f64(xc_family family, const double* left, const double* 
                             uint32_t rows, uint32_t cols, uint32_t n, uint32_t m,
                             const xc_options* opt, xc_stream_t stream);
 
+/* Allocation-free variant of xc_cross_corr_f32 with opt->precision = XC_PREC_FFT64 and
+ * opt->tile = 1 (this library's own FP64 FFT kernels), with the same results.  It has no
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It calls no
+ * hipMalloc, hipFree or allocating library routine: its scratch memory is a fixed 256 MiB
+ * array in the library's code object, one per device the library runs on, held until the
+ * process ends.  rows, cols <= 2048 and the spectra of one pair must fit the array (up to
+ * about 1450 x 1450; 96 MiB at 1024 x 1024); a call with more pairs than fit at once runs in
+ * parts, and a size whose single pair does not fit fails with XC_ERR_UNSUPPORTED.  Calls on
+ * one device share the array, so calls that overlap on one device (other streams or threads)
+ * race.  The first call on a device looks up the array's address and copies the FFT twiddle
+ * tables in (synchronous copies). */
+xc_status xc_cross_corr_noalloc_f32(xc_family family, const float* left, const float* right, float* out,
+                                    uint32_t rows, uint32_t cols, uint32_t n, uint32_t m, xc_stream_t stream);
+
 /* Number of results of a family, or -1 if the counts are invalid for it
  * (one_to_one_validate_input_size etc., src/one_to_one.hpp:20-22,
  * src/one_to_many.hpp:15-17, src/n_to_mn.hpp:23-25, src/n_to_m.hpp:19-21). */
