@@ -72,10 +72,14 @@ direct_plan plan_direct(const family_args& a, const xc_options& opt) {
     t.rm = opt.right_mats_per_thread ? pow2_floor(opt.right_mats_per_thread, 8) : 0;
     t.lm = opt.left_mats_per_thread ? pow2_floor(opt.left_mats_per_thread, 4) : 0;
     t.jb = opt.rows_per_step ? pow2_floor(opt.rows_per_step, 4) : 0;
-    if (!t.rm) t.rm = per_left >= 4 ? 4 : (per_left >= 2 ? 2 : 1);
-    if (!t.lm) t.lm = n_to_m && a.n >= 2 ? 2 : 1;
-    if (!t.ty) t.ty = t.rm * t.lm >= 8 ? 4 : 8;
-    if (!t.jb) t.jb = t.rm * t.lm == 1 ? 4 : (t.rm * t.lm == 2 ? 2 : 1);
+    // Defaults measured on MI355X (one_to_one 128^2, one_to_many 64 x 256^2, n_to_m 16 x 64 of
+    // 64^2): one right matrix: 4 shifts x 4 rows per step; several right matrices per left:
+    // 4 shifts x 2 matrices x 2 rows; n_to_m: 2 shifts x 4 right x 2 left matrices x 2 rows.
+    const bool many_left = n_to_m && a.n >= 2;
+    if (!t.rm) t.rm = many_left ? 4 : (per_left >= 2 ? 2 : 1);
+    if (!t.lm) t.lm = many_left ? 2 : 1;
+    if (!t.ty) t.ty = many_left ? 2 : 4;
+    if (!t.jb) t.jb = t.rm * t.lm == 1 ? 4 : 2;
     // Matrices that do not exist are not tiled.
     t.rm = std::min<int>(t.rm, pow2_floor(per_left, 8));
     t.lm = n_to_m ? std::min<int>(t.lm, pow2_floor(a.n, 4)) : 1;
@@ -83,7 +87,9 @@ direct_plan plan_direct(const family_args& a, const xc_options& opt) {
     direct_plan d{};
     d.t = t;
     const uint32_t SX = 2 * a.cols - 1, SY = 2 * a.rows - 1;
-    uint32_t waves = opt.waves_per_block ? (uint32_t)pow2_floor(opt.waves_per_block, 8) : 4;
+    // 8 waves where one block row spans 256 dx or more and there are enough groups, else 4.
+    uint32_t waves = opt.waves_per_block ? (uint32_t)pow2_floor(opt.waves_per_block, 8)
+                                         : (SX >= 256 && a.pairs >= 8 ? 8 : 4);
     const uint32_t need_x = (SX + 63) / 64;
     d.nwx = 1;
     while (d.nwx * 2 <= waves && d.nwx < need_x) d.nwx *= 2;
