@@ -75,19 +75,21 @@ def pow2ceil(v):
     return p
 
 
-def fft_lengths(H, W):
+def fft_lengths(H, W, pow2_rows=False):
     """Transform lengths of the own FFT method: columns the power of two >= 2H-1 (at least
-    16), rows that or 3 * 2^e (e = 4..9) when shorter (fftk_col_len / fftk_row_len)."""
+    16), rows that or 3 * 2^e (e = 4..9) when shorter (fftk_col_len / fftk_row_len); rows of
+    the power of two only for its tile 2."""
     Pr, Pc = max(16, pow2ceil(2 * H - 1)), max(16, pow2ceil(2 * W - 1))
-    Pc = min([Pc] + [3 * 2 ** e for e in range(4, 10) if 3 * 2 ** e >= 2 * W - 1])
+    if not pow2_rows:
+        Pc = min([Pc] + [3 * 2 ** e for e in range(4, 10) if 3 * 2 ** e >= 2 * W - 1])
     return Pr, Pc
 
 
-def kernel_bytes(name, H, W, n, m, k):
+def kernel_bytes(name, H, W, n, m, k, pow2_rows=False):
     """Algorithmic HBM bytes of one launch of a kernel of the FFT methods (n left, m right
     matrices of H x W, k results): every array the kernel must read is read once and every
     array it must write is written once."""
-    Pr, Pc = fft_lengths(H, W)
+    Pr, Pc = fft_lengths(H, W, pow2_rows)
     Kc = Pc // 2 + 1
     SY, SX = 2 * H - 1, 2 * W - 1
     out = k * SY * SX * 4
@@ -258,7 +260,8 @@ def main():
         sys.exit(1)
     times = kernels_of(lambda: step("auto"), args.steps)
     per_launch = {kname: {"launches": c, "avg_ms": t / c} for kname, (c, t) in times.items()}
-    roof = roofline(times, lambda k: kernel_bytes(k, ROWS, COLS, 1, NUM_RIGHT, NUM_RIGHT),
+    pow2 = bool(chosen_name) and chosen_name[0] == "fft64/2"
+    roof = roofline(times, lambda k: kernel_bytes(k, ROWS, COLS, 1, NUM_RIGHT, NUM_RIGHT, pow2),
                     lambda k: 2.0 * NUM_RIGHT * macs)
     roof["traffic"] = pmc_traffic(roof["kernel"])
     # The FP32 direct kernel's own roofline (the baseline the other methods are compared with).
@@ -325,7 +328,8 @@ def main():
                 "max_norm_err": a.get("max_norm_err"), "methods": res}
         if "ms_per_step" in a:
             t = kernels_of(lambda: x.cross_corr(family, L, R, out=o, precision="auto"), steps)
-            line["roofline"] = roofline(t, lambda kk: kernel_bytes(kk, H, W, n if fam > 1 else 1, m, k),
+            pow2 = line["auto_selected"] == "fft64/2"
+            line["roofline"] = roofline(t, lambda kk: kernel_bytes(kk, H, W, n if fam > 1 else 1, m, k, pow2),
                                         lambda kk: 2.0 * k * float(H * W) ** 2)
         if "ms_per_step" in res.get("fp32", {}) and "ms_per_step" in a:
             line["auto_vs_fp32_direct"] = res["fp32"]["ms_per_step"] / a["ms_per_step"]
@@ -448,7 +452,9 @@ def main():
                 "error_sample": "results 0, 63, 127 at 6 shifts (corners, centre, interior) against the definition",
                 "methods": res}
         t = kernels_of(lambda: x.cross_corr("one_to_many", L, R, out=o5, precision="auto"), 3)
-        line["roofline"] = roofline(t, lambda kk: kernel_bytes(kk, N, N, 1, M, M), lambda kk: 2.0 * M * float(N) ** 4)
+        pow2 = line["auto_selected"] == "fft64/2"
+        line["roofline"] = roofline(t, lambda kk: kernel_bytes(kk, N, N, 1, M, M, pow2),
+                                    lambda kk: 2.0 * M * float(N) ** 4)
         configs["c5_1024"] = line
 
     fp32 = results.get("fp32", {})

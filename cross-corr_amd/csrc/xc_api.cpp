@@ -115,8 +115,9 @@ xc_status xc_cross_corr_f64(xc_family family, const double* left, const double* 
         if (o.precision != XC_PREC_AUTO && o.precision != XC_PREC_FP32 && o.precision != XC_PREC_FFT64)
             xc::fail(XC_ERR_INVALID_ARGUMENT, "double data supports the direct (FP64) and FFT64 methods only");
         xc::family_args a{family, rows, cols, n, m, (uint32_t)xc::num_results(family, n, m)};
-        if (o.precision == XC_PREC_FFT64 && o.tile == 1)
-            xc::run_fftk_f64(a, left, right, out, o.split, (hipStream_t)stream);
+        if (o.precision == XC_PREC_AUTO) o = xc::auto_options_f64(a, left, right, out, (hipStream_t)stream);
+        if (o.precision == XC_PREC_FFT64 && (o.tile == 1 || o.tile == 2))
+            xc::run_fftk_f64(a, left, right, out, o.split, o.tile == 2, (hipStream_t)stream);
         else if (o.precision == XC_PREC_FFT64)
             xc::run_fft_f64(a, left, right, out, true, (hipStream_t)stream);
         else

@@ -271,7 +271,7 @@ xc_batch_plan* batch_plan_create(const xc_pair* pairs, uint32_t num_pairs, const
     if (!x3 && kind != XC_PREC_BF16 && kind != XC_PREC_FP16)
         fail(XC_ERR_INVALID_ARGUMENT, "batch plans run the MFMA methods (bf16x3, fp16x3, bf16, fp16) or FFT64 tile 1");
     if (opt.border > 512) fail(XC_ERR_INVALID_ARGUMENT, "border threshold above 512");
-    const uint32_t thresh = x3 ? (opt.border ? opt.border : batch_default_border()) : 0;
+    const uint32_t thresh = x3 ? (opt.border ? opt.border : batch_default_border(kind)) : 0;
 
     std::vector<batch_pair_desc> descs(num_pairs);
     std::vector<uint32_t> tables;

@@ -86,12 +86,13 @@ void run_family_f32(const family_args& a, const float* L, const float* R, float*
         launch_mfma_f32(a, L, R, O, (int)opt.precision, opt.border, opt.tile, s);
         return;
     case XC_PREC_FFT64:
-        if (opt.tile == 1)
-            run_fftk_f32(a, L, R, O, opt.split, s);
+        if (opt.tile == 1 || opt.tile == 2)
+            run_fftk_f32(a, L, R, O, opt.split, opt.tile == 2, s);
         else if (opt.tile == 0)
             run_fft_f32(a, L, R, O, true, true, s);
         else
-            fail(XC_ERR_INVALID_ARGUMENT, "FFT64 tile must be 0 (rocFFT) or 1 (own FFT kernels)");
+            fail(XC_ERR_INVALID_ARGUMENT,
+                 "FFT64 tile must be 0 (rocFFT), 1 (own FFT kernels) or 2 (own FFT, power-of-two rows)");
         return;
     case XC_PREC_FFT32:
         run_fft_f32(a, L, R, O, true, false, s);

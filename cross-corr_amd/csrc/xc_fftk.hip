@@ -904,84 +904,124 @@ __device__ __forceinline__ void radix3_pass(const double2* buf, const r3_roots<L
     }
 }
 
-// rows_fwd for P = 3N: block (matrix, group of 2T rows r0 = 2T g ...), otherwise as
-// rows_fwd_slab_kernel (matrices, offsets, sizes, output S).
+// rows_fwd for P = 3N: a persistent grid as rows_fwd_regs_kernel (items: matrix, group of 2T
+// rows; each XCD walks one contiguous eighth of the items; the next item's rows are loaded
+// into registers while the current one is transformed and stored), matrices, offsets, sizes
+// and output S as rows_fwd_slab_kernel.
 template <typename Tin, int LOGN>
 __global__ __launch_bounds__(rows3<LOGN>::NT) void rows_fwd3_kernel(
     const Tin* __restrict__ XL, const Tin* __restrict__ XR, const uint64_t* __restrict__ moff,
-    const uint2* __restrict__ mdims, uint32_t nl, uint32_t H, uint32_t W, uint32_t groups, double2* __restrict__ SL,
-    double2* __restrict__ SR, const double2* __restrict__ tw, const double2* __restrict__ tw3) {
+    const uint2* __restrict__ mdims, uint32_t nl, uint32_t H, uint32_t W, uint32_t groups, uint32_t items,
+    double2* __restrict__ SL, double2* __restrict__ SR, const double2* __restrict__ tw,
+    const double2* __restrict__ tw3) {
     using G = rows3<LOGN>;
     constexpr int N = G::N, P = G::P, Q = G::Q, T = G::T;
     constexpr uint32_t Kc = P / 2 + 1;
     extern __shared__ double2 buf[];
-    const uint32_t matg = blockIdx.x / groups, g = blockIdx.x - matg * groups;
-    const bool left = matg < nl;
-    const uint32_t mat = left ? matg : matg - nl;
-    uint32_t Hm = H, Wm = W;
-    if (mdims) {
-        const uint2 d = mdims[matg];
-        Hm = d.x;
-        Wm = d.y;
-    }
-    const uint32_t r0 = g * (2 * T);
-    if (r0 >= Hm) return;  // a row group beyond a smaller matrix
-    const Tin* Xm = (left ? XL : XR) + (moff ? moff[matg] : (size_t)mat * H * W);
     const regs_pos<LOGN> ps;
     const int sub = ps.t, tt = sub / 3, m = sub - 3 * tt, i = ps.i;
-    const uint32_t a = r0 + 2 * tt, b = a + 1;
-    const bool oka = a < Hm, okb = b < Hm;
-    const Tin* xa = Xm + (oka ? (size_t)a * Wm : 0);
-    const Tin* xb = Xm + (okb ? (size_t)b * Wm : 0);
     const regs_roots<LOGN> rr = load_roots<LOGN>(i, tw);
     const r3_roots<LOGN> w3 = load_r3_roots<LOGN>(tw3);
+    const uint32_t nx = gridDim.x >= 8 ? 8 : 1;
+    const uint32_t x = blockIdx.x % nx, kb = blockIdx.x / nx, bpx = gridDim.x / nx;
+    const uint32_t per = (items + nx - 1) / nx;
+    const uint32_t end = min((x + 1) * per, items);
+    struct where {
+        const Tin* xa;
+        const Tin* xb;
+        uint32_t Hm, Wm, r0, mat;
+        bool left, oka, okb;
+    };
+    auto locate = [&](uint32_t it) {
+        where w;
+        const uint32_t matg = it / groups, g = it - matg * groups;
+        w.left = matg < nl;
+        w.mat = w.left ? matg : matg - nl;
+        w.r0 = g * (2 * T);
+        w.Hm = H;
+        w.Wm = W;
+        if (mdims) {
+            const uint2 d = mdims[matg];
+            w.Hm = d.x;
+            w.Wm = d.y;
+        }
+        const Tin* Xm = (w.left ? XL : XR) + (moff ? moff[matg] : (size_t)w.mat * H * W);
+        const uint32_t a = w.r0 + 2 * tt, b = a + 1;
+        w.oka = a < w.Hm;
+        w.okb = b < w.Hm;
+        w.xa = Xm + (w.oka ? (size_t)a * w.Wm : 0);
+        w.xb = Xm + (w.okb ? (size_t)b * w.Wm : 0);
+        return w;
+    };
+    // element n = i + Q r of sub-sequence m is column 3n + m; loads from clamped addresses
+    auto fetch = [&](const where& w, Tin (&va)[8], Tin (&vb)[8]) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const uint32_t c = 3u * (uint32_t)(i + Q * r) + (uint32_t)m, cc = c < w.Wm ? c : w.Wm - 1;
+            va[r] = w.xa[cc];
+            vb[r] = w.xb[cc];
+        }
+    };
+    uint32_t it = x * per + kb;
+    if (it >= end) return;
+    where cur = locate(it);
     Tin va[8], vb[8];
+    fetch(cur, va, vb);
+    while (true) {
+        const uint32_t nxt = it + bpx;
+        const bool more = nxt < end;
+        double2 u[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {  // element n = i + Q r of sub-sequence m: column 3n + m
-        const uint32_t c = 3u * (uint32_t)(i + Q * r) + (uint32_t)m, cc = c < Wm ? c : Wm - 1;
-        va[r] = xa[cc];
-        vb[r] = xb[cc];
-    }
-    double2 u[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        const bool in = 3u * (uint32_t)(i + Q * r) + (uint32_t)m < Wm;
-        u[r] = make_double2(oka && in ? (double)va[r] : 0.0, okb && in ? (double)vb[r] : 0.0);
-    }
-    fft_regs<LOGN>(u, buf + at<LOGN>(ps.t0, 0), rr, sub - ps.t0, i);
-#pragma unroll
-    for (int r = 0; r < 8; ++r) buf[at<LOGN>(sub, i + Q * r)] = u[r];
-    __syncthreads();
-    double2 X[G::NI][3];
-    radix3_pass<LOGN>(buf, w3, X);
-    __syncthreads();  // X overwrites the sub-spectra
-#pragma unroll
-    for (int q = 0; q < G::NI; ++q) {
-        const int idx = (int)threadIdx.x + G::NT * q;
-        if (idx < T * N) {
-            const int t2 = idx >> LOGN, k = idx & (N - 1);
-#pragma unroll
-            for (int j = 0; j < 3; ++j) buf[xat<LOGN>(t2, k + N * j)] = X[q][j];
+        for (int r = 0; r < 8; ++r) {
+            const bool in = 3u * (uint32_t)(i + Q * r) + (uint32_t)m < cur.Wm;
+            u[r] = make_double2(cur.oka && in ? (double)va[r] : 0.0, cur.okb && in ? (double)vb[r] : 0.0);
         }
-    }
-    __syncthreads();
-    // split into the two rows' half spectra (as split_rows), column-major into S
-    double2* Sm = (left ? SL : SR) + (size_t)mat * Kc * H;
-    for (int idx = threadIdx.x; idx < (int)Kc * 2 * T; idx += G::NT) {
-        const int k = idx >> (G::LOGT + 1), rw = idx & (2 * T - 1);
-        const uint32_t r = r0 + rw;
-        if (r >= Hm) continue;
-        const int t2 = rw >> 1;
-        const double2 zk = buf[xat<LOGN>(t2, k)];
-        const double2 zn = cconj(buf[xat<LOGN>(t2, k == 0 ? 0 : P - k)]);
-        double2 v;
-        if ((rw & 1) == 0) {
-            v = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y + zn.y));
-        } else {
-            const double2 d = mul_mi(csub(zk, zn));
-            v = make_double2(0.5 * d.x, 0.5 * d.y);
+        where nw{};
+        if (more) {
+            nw = locate(nxt);
+            fetch(nw, va, vb);
         }
-        Sm[(size_t)k * H + r] = v;
+        if (cur.r0 < cur.Hm) {  // block-uniform: a row group beyond a smaller matrix is skipped
+            fft_regs<LOGN>(u, buf + at<LOGN>(ps.t0, 0), rr, sub - ps.t0, i);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) buf[at<LOGN>(sub, i + Q * r)] = u[r];
+            __syncthreads();
+            double2 X[G::NI][3];
+            radix3_pass<LOGN>(buf, w3, X);
+            __syncthreads();  // X overwrites the sub-spectra
+#pragma unroll
+            for (int q = 0; q < G::NI; ++q) {
+                const int idx = (int)threadIdx.x + G::NT * q;
+                if (idx < T * N) {
+                    const int t2 = idx >> LOGN, k = idx & (N - 1);
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) buf[xat<LOGN>(t2, k + N * j)] = X[q][j];
+                }
+            }
+            __syncthreads();
+            // split into the two rows' half spectra (as split_rows), column-major into S
+            double2* Sm = (cur.left ? SL : SR) + (size_t)cur.mat * Kc * H;
+            for (int idx = threadIdx.x; idx < (int)Kc * 2 * T; idx += G::NT) {
+                const int k = idx >> (G::LOGT + 1), rw = idx & (2 * T - 1);
+                const uint32_t r = cur.r0 + rw;
+                if (r >= cur.Hm) continue;
+                const int t2 = rw >> 1;
+                const double2 zk = buf[xat<LOGN>(t2, k)];
+                const double2 zn = cconj(buf[xat<LOGN>(t2, k == 0 ? 0 : P - k)]);
+                double2 v;
+                if ((rw & 1) == 0) {
+                    v = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y + zn.y));
+                } else {
+                    const double2 d = mul_mi(csub(zk, zn));
+                    v = make_double2(0.5 * d.x, 0.5 * d.y);
+                }
+                Sm[(size_t)k * H + r] = v;
+            }
+            __syncthreads();  // the next item's transform reuses the slab
+        }
+        if (!more) break;
+        it = nxt;
+        cur = nw;
     }
 }
 
@@ -1201,13 +1241,13 @@ struct fftk_geom {
     size_t s_bytes, col_bytes, z_bytes;
 };
 
-fftk_geom geom_of(uint32_t H, uint32_t W) {
+fftk_geom geom_of(uint32_t H, uint32_t W, bool pow2_rows = false) {
     if (H == 0 || W == 0 || 2 * H - 1 > (uint32_t)kSlab || 2 * W - 1 > (uint32_t)kSlab)
         fail(XC_ERR_UNSUPPORTED, "own-FFT method: rows and cols must be 1..2048");
     fftk_geom g{};
     g.logPr = std::max(4, ilog2_ceil(2 * H - 1));
     g.Pr = 1u << g.logPr;
-    g.Pc = fftk_row_len(W);
+    g.Pc = pow2_rows ? fftk_col_len(W) : fftk_row_len(W);
     g.f3 = (g.Pc & (g.Pc - 1)) != 0;
     g.logN = g.f3 ? ilog2_ceil(g.Pc / 3) : 0;
     g.logPc = g.f3 ? 0 : ilog2_ceil(g.Pc);
@@ -1278,7 +1318,7 @@ template <typename Tio>
 void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fftk_ws& ws, hipStream_t s,
                 bool timing) {
     const uint32_t H = j.H, W = j.W;
-    const fftk_geom g = geom_of(H, W);
+    const fftk_geom g = geom_of(H, W, j.pow2_rows);
     const double2* tw = twiddles();
     double2* SR = static_cast<double2*>(ws.sr);
     double2* SL = static_cast<double2*>(ws.sl);
@@ -1297,8 +1337,10 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
                     fail(XC_ERR_UNSUPPORTED, "own-FFT method: inconsistent 3N-point row geometry");
                 auto k = rows_fwd3_kernel<Tio, LOGN>;
                 allow_lds(k, g.v1.lds);
-                launch_timed("fftk_rows_fwd", timing, k, dim3((j.nl + j.nr) * g.g1), dim3(G3::NT), g.v1.lds, s, L, R,
-                             j.moff, mdims, j.nl, H, W, g.g1, SL, SR, tw, tw3);
+                const uint32_t items = (j.nl + j.nr) * g.g1;
+                const uint32_t grid = resident_grid(reinterpret_cast<const void*>(k), G3::NT, g.v1.lds, items);
+                launch_timed("fftk_rows_fwd", timing, k, dim3(grid), dim3(G3::NT), g.v1.lds, s, L, R, j.moff, mdims,
+                             j.nl, H, W, g.g1, items, SL, SR, tw, tw3);
             }
         });
     } else
@@ -1362,9 +1404,11 @@ void launch_job(const fftk_job& j, const Tio* L, const Tio* R, Tio* O, const fft
 }
 
 template <typename Tio>
-void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, uint32_t max_chunk, hipStream_t s) {
+void run_fftk(const family_args& a, const Tio* L, const Tio* R, Tio* O, uint32_t max_chunk, bool pow2_rows,
+              hipStream_t s) {
     fftk_job j{};
     j.max_chunk = max_chunk;
+    j.pow2_rows = pow2_rows;
     j.H = a.rows;
     j.W = a.cols;
     j.nl = a.family == XC_ONE_TO_ONE || a.family == XC_ONE_TO_MANY ? 1 : a.n;
@@ -1546,7 +1590,7 @@ uint32_t fftk_row_len(uint32_t cols) {
 }
 
 fftk_ws_sizes fftk_sizes(const fftk_job& j) {
-    const fftk_geom g = geom_of(j.H, j.W);
+    const fftk_geom g = geom_of(j.H, j.W, j.pow2_rows);
     fftk_ws_sizes z{};
     // Z is processed in chunks of pairs of at most ~2 GiB.
     z.chunk = (uint32_t)std::max<size_t>(1, std::min<size_t>(j.pairs, (size_t(2) << 30) / g.z_bytes));
@@ -1567,13 +1611,14 @@ void fftk_launch_f32(const fftk_job& j, const float* L, const float* R, float* O
     launch_job<float>(j, L, R, O, ws, s, false);
 }
 
-void run_fftk_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t max_chunk, hipStream_t s) {
-    run_fftk<float>(a, L, R, O, max_chunk, s);
+void run_fftk_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t max_chunk, bool pow2_rows,
+                  hipStream_t s) {
+    run_fftk<float>(a, L, R, O, max_chunk, pow2_rows, s);
 }
 
 void run_fftk_f64(const family_args& a, const double* L, const double* R, double* O, uint32_t max_chunk,
-                  hipStream_t s) {
-    run_fftk<double>(a, L, R, O, max_chunk, s);
+                  bool pow2_rows, hipStream_t s) {
+    run_fftk<double>(a, L, R, O, max_chunk, pow2_rows, s);
 }
 
 }  // namespace xc

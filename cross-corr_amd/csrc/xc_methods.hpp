@@ -11,7 +11,7 @@ namespace xc {
 void* workspace(int slot, size_t bytes, hipStream_t s);
 enum : int {
     WS_FFT_PAD = 0, WS_FFT_SPEC = 1, WS_FFT_OUT = 2, WS_FFT_RES = 3, WS_SPLIT = 4, WS_BATCH = 5,
-    WS_FK_SR = 6, WS_FK_SL = 7, WS_FK_LHAT = 8, WS_FK_Z = 9
+    WS_FK_SR = 6, WS_FK_SL = 7, WS_FK_LHAT = 8, WS_FK_Z = 9, WS_FRAME = 10
 };
 
 // Dispatch of xc_cross_corr_f32 on opt.precision (xc_precision).
@@ -38,7 +38,7 @@ struct batch_item {
 uint32_t batch_tiles_x(uint32_t cols);
 uint32_t batch_tiles_y(uint32_t rows);
 uint32_t batch_frame_chunk();
-uint32_t batch_default_border();
+uint32_t batch_default_border(int kind);
 uint32_t frame_table_host(uint32_t rows, uint32_t cols, uint32_t thresh, std::vector<uint32_t>& start);
 const void* batch_kernel_ptr(int kind);
 uint32_t batch_lds_bytes(int kind);
@@ -75,6 +75,8 @@ struct fftk_job {
     // Largest number of pairs per launch of the column and inverse-row kernels (0: as
     // many as fit in 2 GiB of Z).
     uint32_t max_chunk = 0;
+    // Row transforms of the power of two >= 2W-1 even where 3 * 2^n is shorter.
+    bool pow2_rows = false;
 };
 struct fftk_ws_sizes {
     size_t sr, sl, lh, z;
@@ -96,10 +98,12 @@ void fftk_launch_f32(const fftk_job& j, const float* L, const float* R, float* O
 // The own FFT without any allocation: scratch memory is a fixed array of the code object
 // (rows, cols <= 2048; one pair's spectra must fit it).
 void run_fftk_noalloc_f32(const family_args& a, const float* L, const float* R, float* O, hipStream_t s);
-// max_chunk: pairs per Z chunk as fftk_job::max_chunk (0 = default).
-void run_fftk_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t max_chunk, hipStream_t s);
-void run_fftk_f64(const family_args& a, const double* L, const double* R, double* O, uint32_t max_chunk,
+// max_chunk: pairs per Z chunk as fftk_job::max_chunk (0 = default); pow2_rows as
+// fftk_job::pow2_rows.
+void run_fftk_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t max_chunk, bool pow2_rows,
                   hipStream_t s);
+void run_fftk_f64(const family_args& a, const double* L, const double* R, double* O, uint32_t max_chunk,
+                  bool pow2_rows, hipStream_t s);
 
 void check_scatter(const void* src, const void* dst, uint32_t src_rows, uint32_t src_cols, uint32_t dst_rows,
                    uint32_t dst_cols, uint32_t dst_row, uint32_t dst_col);

@@ -34,7 +34,12 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kMaxBorder = 512;       // largest accepted border threshold
-constexpr uint32_t kDefaultBorder = 512;  // overlap area below which bf16x3 outputs are recomputed
+// Overlap area below which split-precision outputs are recomputed exactly.  Measured maximum
+// normalised error of the tiles alone per overlap-area octave (u01, u11 and normal data, 32^2
+// to 256^2): bf16x3 falls from 2e-5 at area 1 to 2.1e-6 at 256..511 and 1.6e-6 at 512..1023;
+// fp16x3 (22 significant bits per operand) is at most 1.1e-6 from area 4 on, 2e-5 below.
+constexpr uint32_t kDefaultBorder = 512;        // bf16x3
+constexpr uint32_t kDefaultBorderFp16x3 = 16;   // fp16x3
 
 template <int KIND>
 struct split_traits;
@@ -85,12 +90,14 @@ __device__ __forceinline__ f32x16 mma(const uint4& a, const uint4& b, f32x16 c) 
                                                        c, 0, 0, 0);
 }
 
-// LDS geometry (bytes) of one block; KC columns c' per pass over the left rows.
-template <int WX, int WY, int PLANES, int KC>
+// LDS geometry (bytes) of one block; KC columns c' per pass over the left rows, JR left rows
+// per step (between two block barriers).
+template <int WX, int WY, int PLANES, int KC, int JR = 1>
 struct geom {
     static constexpr int BX = 64 * WX;                    // dx per block
     static constexpr int BY = 64 * WY;                    // dy per block
-    static constexpr int NSLOT = BY + 1;                  // ring slots (right rows)
+    // ring slots (right rows): the BY + JR - 1 rows a step reads and the JR the next step adds
+    static constexpr int NSLOT = BY + 2 * JR - 1;
     static constexpr int RS = KC * 2 + 16;               // ring row stride: an odd multiple of 16 B
     static constexpr int LENA = KC + BX + 64;            // staged left-row elements per copy
     // Copy stride = 32 mod 128: the 16 lanes of one ds_read_b64 cycle (4 alignments x 4
@@ -102,7 +109,7 @@ struct geom {
     static constexpr int SS = ((PLANES * RS / 16) % 2) ? PLANES * RS : PLANES * RS + 16;
     static constexpr int RING = NSLOT * SS;
     static constexpr int LBUF = PLANES * 4 * CS;          // one left-row buffer (4 shifted copies)
-    static constexpr int BYTES = RING + 2 * LBUF;
+    static constexpr int BYTES = RING + 2 * JR * LBUF;
 };
 
 __device__ __forceinline__ int ring_slot(int row, int nslot) {
@@ -223,13 +230,16 @@ __device__ __forceinline__ void chunk_range_dispatch(int mask, f32x16 (&acc)[2][
 // budget is sized for.
 // One block tile (dx tile tx, dy tile ty) of the pair (Lp, Rp) -> Op, both H x W.  Outputs
 // whose overlap area is below skip_area are left to the border-frame code (0: write all).
-template <int KIND, int WX, int WY, int KC, int FLUSH, bool KAHAN, int OCC>
+template <int KIND, int WX, int WY, int KC, int FLUSH, bool KAHAN, int OCC, int JR = 1>
 __device__ __forceinline__ void mfma_tile(const float* __restrict__ Lp, const float* __restrict__ Rp,
                                           float* __restrict__ Op, const int H, const int W, const uint32_t tx,
                                           const uint32_t ty, const uint32_t skip_area, char* smem) {
     using S = split_traits<KIND>;
     constexpr int P = S::planes;
-    using G = geom<WX, WY, P, KC>;
+    using G = geom<WX, WY, P, KC, JR>;
+    static_assert(JR == 1 || JR == 2, "one or two left rows per step");
+    static_assert(FLUSH % JR == 0, "flushes fall on step boundaries");
+    static_assert(JR * (KC / 4) <= 256, "one thread per staged right-row quad");
     constexpr bool PIPE = OCC == 1;  // two waves per SIMD hide LDS latency without it
     char* ring = smem;
     char* lbuf = smem + G::RING;
@@ -323,13 +333,16 @@ __device__ __forceinline__ void mfma_tile(const float* __restrict__ Lp, const fl
 
     for (int c0 = (cb_lo / KC) * KC; c0 < cb_hi; c0 += KC) {
         const int cbase = c0 + dxb;  // left column at staged position 0
-        // Prefill: right rows r_lo - dy for dy in the block (the ring), left row r_lo.
-        for (int idx = threadIdx.x; idx < G::BY * RQ; idx += 256) {
+        // Prefill: right rows r - dy for r in [r_lo, r_lo + JR) and dy in the block (the ring),
+        // left rows r_lo .. r_lo + JR - 1 into buffers 0 .. JR - 1.
+        for (int idx = threadIdx.x; idx < (G::BY + JR - 1) * RQ; idx += 256) {
             const int ro = idx / RQ, j = idx - (idx / RQ) * RQ;
-            const int row = r_lo - dyb - ro;
+            const int row = r_lo + JR - 1 - dyb - ro;
             put_right(row, j, load_right(row, c0, j));
         }
-        for (int item = threadIdx.x; item < LITEMS; item += 256) put_left(0, item, load_left(r_lo, cbase, item));
+        for (int i = 0; i < JR; ++i)
+            for (int item = threadIdx.x; item < LITEMS; item += 256)
+                put_left(i, item, load_left(r_lo + i, cbase, item));
         __syncthreads();
 
         // Wave's K chunks of this pass, and those of each dx sub-tile (offsets from c0,
@@ -348,20 +361,28 @@ __device__ __forceinline__ void mfma_tile(const float* __restrict__ Lp, const fl
             }
         }
 
-        for (int r = r_lo; r < r_hi; ++r) {
-            const int cur = (r - r_lo) & 1;
-            const bool more = r + 1 < r_hi;
-            // Prefetch next step's data into registers.
-            float4 pl[LPT];
+        for (int r0 = r_lo; r0 < r_hi; r0 += JR) {
+            const int cur = ((r0 - r_lo) / JR) & 1;  // buffer set of this step's left rows
+            const bool more = r0 + JR < r_hi;
+            // Prefetch the next step's rows into registers.
+            float4 pl[JR][LPT];
             float4 pr = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-            for (int i = 0; i < LPT; ++i) {
-                const int item = threadIdx.x + i * 256;
-                pl[i] = (more && item < LITEMS) ? load_left(r + 1, cbase, item) : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-            if (more && threadIdx.x < RQ) pr = load_right(r + 1 - dyb, c0, threadIdx.x);
+            for (int i = 0; i < JR; ++i)
+#pragma unroll
+                for (int k = 0; k < LPT; ++k) {
+                    const int item = threadIdx.x + k * 256;
+                    pl[i][k] = (more && item < LITEMS && r0 + JR + i < r_hi) ? load_left(r0 + JR + i, cbase, item)
+                                                                            : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+            if (more && threadIdx.x < JR * RQ)
+                pr = load_right(r0 + JR + (int)threadIdx.x / RQ - dyb, c0, (int)threadIdx.x % RQ);
 
-            // Sub-tiles (32 dy x 32 dx) this step touches: left row r meets dy sub-tile u,
+#pragma unroll
+            for (int ri = 0; ri < JR; ++ri) {
+            const int r = r0 + ri;
+            if (r >= r_hi) break;
+            // Sub-tiles (32 dy x 32 dx) this row touches: left row r meets dy sub-tile u,
             // chunk [c0+qq, c0+qq+16) meets the columns of dx sub-tile s.
             bool act_u[2];
 #pragma unroll
@@ -370,7 +391,7 @@ __device__ __forceinline__ void mfma_tile(const float* __restrict__ Lp, const fl
                 act_u[u] = lo <= hi && r >= max(0, lo) && r < min(H, H + hi);
             }
             if (q_lo < q_hi && (act_u[0] || act_u[1])) {
-                const char* lb = lbuf + cur * G::LBUF;
+                const char* lb = lbuf + (cur * JR + ri) * G::LBUF;
                 // Ring rows of this lane for the two dy sub-tiles: row = r - (dyw + 32u + li32).
                 const char* arow[2];
 #pragma unroll
@@ -407,9 +428,11 @@ __device__ __forceinline__ void mfma_tile(const float* __restrict__ Lp, const fl
                         chunk_range_dispatch<KIND, P, PIPE>(mask, acc, arow, bcol, q0, q1, G::RS, G::CS);
                 }
             }
+            }  // rows of the step
+
             // Every FLUSH steps (and at the end of a pass) the step sums are added to the
             // totals with a compensated (Kahan) add.
-            if (((r - r_lo + 1) % FLUSH) == 0 || r + 1 == r_hi) {
+            if (((r0 - r_lo + JR) % FLUSH) == 0 || !more) {
 #pragma unroll
                 for (int u = 0; u < 2; ++u)
 #pragma unroll
@@ -429,11 +452,14 @@ __device__ __forceinline__ void mfma_tile(const float* __restrict__ Lp, const fl
 
             if (more) {
 #pragma unroll
-                for (int i = 0; i < LPT; ++i) {
-                    const int item = threadIdx.x + i * 256;
-                    if (item < LITEMS) put_left(cur ^ 1, item, pl[i]);
-                }
-                if (threadIdx.x < RQ) put_right(r + 1 - dyb, threadIdx.x, pr);
+                for (int i = 0; i < JR; ++i)
+#pragma unroll
+                    for (int k = 0; k < LPT; ++k) {
+                        const int item = threadIdx.x + k * 256;
+                        if (item < LITEMS) put_left((cur ^ 1) * JR + i, item, pl[i][k]);
+                    }
+                if (threadIdx.x < JR * RQ)
+                    put_right(r0 + JR + (int)threadIdx.x / RQ - dyb, (int)threadIdx.x % RQ, pr);
             }
             __syncthreads();
         }
@@ -457,7 +483,7 @@ __device__ __forceinline__ void mfma_tile(const float* __restrict__ Lp, const fl
     }
 }
 
-template <int KIND, int WX, int WY, int KC, int FLUSH, bool KAHAN, int OCC>
+template <int KIND, int WX, int WY, int KC, int FLUSH, bool KAHAN, int OCC, int JR>
 __global__ __launch_bounds__(256, OCC) void mfma_kernel(const float* __restrict__ L, const float* __restrict__ R,
                                                       float* __restrict__ O, int family, uint32_t rows,
                                                       uint32_t cols, uint32_t n, uint32_t m, uint32_t tiles_x,
@@ -471,7 +497,7 @@ __global__ __launch_bounds__(256, OCC) void mfma_kernel(const float* __restrict_
     const uint32_t k = bid / tiles_y;
     uint32_t li, ri;
     pair_of(family, n, m, k, li, ri);
-    mfma_tile<KIND, WX, WY, KC, FLUSH, KAHAN, OCC>(L + (size_t)li * H * W, R + (size_t)ri * H * W,
+    mfma_tile<KIND, WX, WY, KC, FLUSH, KAHAN, OCC, JR>(L + (size_t)li * H * W, R + (size_t)ri * H * W,
                                                    O + (size_t)k * (2 * W - 1) * (2 * H - 1), H, W, tx, ty,
                                                    skip_area, smem);
 }
@@ -483,34 +509,18 @@ struct frame_table {
     uint32_t start[kMaxBorder + 1];
 };
 
-// Exact FP32 value of one output (dy, dx) with overlap h x w, computed by one wave: the lanes
-// cover the h*w products (whole rows when w < 64), each lane sums its share and the 64 lane
-// sums are added by a butterfly reduction; lane 0 stores.
-__device__ __forceinline__ void frame_output(const float* __restrict__ Lp, const float* __restrict__ Rp,
-                                             float* __restrict__ Op, int H, int W, int dy, int dx, int h, int w,
-                                             int lane) {
+// Value of one output (dy, dx) with overlap h x w, computed by one thread: products of FP32
+// values are exact in FP64 and their FP64 sum carries a relative error below h*w*2^-53, so the
+// result is the exact sum rounded once to FP32.
+__device__ __forceinline__ float frame_value(const float* __restrict__ Lp, const float* __restrict__ Rp, int W,
+                                             int dy, int dx, int h, int w) {
     const int r0 = max(0, dy), c0 = max(0, dx);
-    int rs, cl, rstep, cstep;
-    if (w >= 64) {
-        rs = 0;
-        cl = lane;
-        rstep = 1;
-        cstep = 64;
-    } else {
-        rstep = 64 / w;
-        rs = lane / w;
-        cl = lane - rs * w;
-        cstep = w;
-        if (rs >= rstep) rs = h;  // idle lane
-    }
     const float* __restrict__ lp = Lp + (size_t)r0 * W + c0;
     const float* __restrict__ rp = Rp + (size_t)(r0 - dy) * W + (c0 - dx);
-    float acc = 0.f;
-    for (int r = rs; r < h; r += rstep)
-        for (int c = cl; c < w; c += cstep) acc = fmaf(lp[(size_t)r * W + c], rp[(size_t)r * W + c], acc);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) Op[(size_t)(dy + H - 1) * (2 * W - 1) + (dx + W - 1)] = acc;
+    double acc = 0.0;
+    for (int r = 0; r < h; ++r, lp += W, rp += W)
+        for (int c = 0; c < w; ++c) acc = fma((double)lp[c], (double)rp[c], acc);
+    return (float)acc;
 }
 
 // (h, w) of frame index hw from a table with start[0..nh] (largest h with start[h-1] <= hw).
@@ -525,42 +535,151 @@ __device__ __forceinline__ void frame_hw(const uint32_t* start, uint32_t nh, uin
     w = (int)(hw - start[lo]) + 1;
 }
 
-// Exact FP32 recomputation of the outputs whose overlap area (H-|dy|)*(W-|dx|) is below the
-// threshold: one wave per (pair, overlap size (h, w), quadrant).  The wave's lanes cover the
-// h*w products (whole rows when w < 64), each lane sums its share in FP32 and the 64 lane
-// sums are added by a butterfly reduction.
-__global__ __launch_bounds__(256) void frame_kernel(const float* __restrict__ L, const float* __restrict__ R,
-                                                    float* __restrict__ O, int family, uint32_t rows,
-                                                    uint32_t cols, uint32_t n, uint32_t m, uint32_t pairs,
-                                                    frame_table tab) {
-    const int H = (int)rows, W = (int)cols;
-    const int SY = 2 * H - 1, SX = 2 * W - 1;
-    const int lane = threadIdx.x & 63;
-    const uint32_t o = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t hw = o >> 2, q = o & 3;
-    if (hw >= tab.start[tab.nh]) return;
-    int h, w;
-    frame_hw(tab.start, tab.nh, hw, h, w);
-    const int dy = (q & 1) ? -(H - h) : (H - h);
-    const int dx = (q & 2) ? -(W - w) : (W - w);
-    if (((q & 1) && dy == 0) || ((q & 2) && dx == 0)) return;  // the same output as another quadrant
-    for (uint32_t k = blockIdx.y; k < pairs; k += gridDim.y) {
-        uint32_t li, ri;
-        pair_of(family, n, m, k, li, ri);
-        frame_output(L + (size_t)li * H * W, R + (size_t)ri * H * W, O + (size_t)k * SX * SY, H, W, dy, dx, h, w,
-                     lane);
+// Exact recomputation of the outputs whose overlap area h*w (h = H-|dy|, w = W-|dx|) is
+// below the threshold, in tasks: a task is (quadrant, height h, group of up to kFrameGroup
+// consecutive widths), and the lanes of a wave run one task for LP consecutive pairs (64/LP
+// tasks per wave), so that every lane of a task has the same trip count.  The matrices are
+// read from padded interleaved copies (element (r, c) of matrix i at ((r * Wp + c) * count + i),
+// Wp = W + kFramePad, zero for c >= W), so that the lanes of a task, which take consecutive
+// pairs, read consecutive addresses, and reads past a row end need no test.
+//
+// Every quadrant reads as out(s) = sum_{i<h} sum_{c<w} A[ra+i][s+c] * B[rb+i][c] with s = W-w:
+// (A, B) = (L, R) where dx >= 0 and (R, L) where dx < 0; rows ra, rb are H-h and 0 in the
+// order (L, R) where dy >= 0, 0 and H-h where dy < 0.  The group's outputs (s = s0 + k,
+// k < kFrameGroup) take the columns c in steps of kFrameGroup: step (i, g) reads A[ra+i][s0 + 8g
+// + j] for j < 2 kFrameGroup and B[rb+i][8g + j] for j < kFrameGroup, and adds A[..][s0 + 8g +
+// j + k] * B[..][8g + j] to output k; A is zero past the row end, which ends each output's sum
+// at its own width.  The steps run as one flat sequence over (i, g) with the next step's
+// values loaded while the current one is summed.  Products of FP32 values are exact in FP64,
+// so each output is its exact sum rounded once.
+constexpr int kFrameGroup = 8;
+constexpr int kFramePad = 2 * kFrameGroup;
+
+// Padded interleaved copy: X holds count matrices of rows x cols; Xt[(r * (cols + kFramePad) +
+// c) * count + i] = X[i][r][c], zero for c >= cols.  32 x 32 tiles (elements x matrices)
+// through LDS, so that reads and writes are both coalesced.
+__global__ __launch_bounds__(256) void interleave_kernel(const float* __restrict__ X, float* __restrict__ Xt,
+                                                         uint32_t count, uint32_t cols, uint64_t padded_area) {
+    __shared__ float tile[32][33];
+    const uint64_t e0 = (uint64_t)blockIdx.x * 32;  // padded element index r * Wp + c
+    const uint32_t i0 = blockIdx.y * 32;
+    const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const uint32_t wp = cols + kFramePad;
+    for (uint32_t k = ty; k < 32; k += 8) {  // matrix i0 + k, padded element e0 + tx
+        const uint32_t i = i0 + k;
+        const uint64_t e = e0 + tx;
+        const uint64_t r = e / wp;
+        const uint32_t c = (uint32_t)(e - r * wp);
+        tile[k][tx] = (i < count && e < padded_area && c < cols) ? X[((uint64_t)i * (padded_area / wp) + r) * cols + c]
+                                                                 : 0.f;
+    }
+    __syncthreads();
+    for (uint32_t k = ty; k < 32; k += 8) {  // padded element e0 + k of matrices i0 + tx
+        const uint32_t i = i0 + tx;
+        const uint64_t e = e0 + k;
+        if (i < count && e < padded_area) Xt[e * count + i] = tile[tx][k];
     }
 }
 
-template <int KIND, int WX, int WY, int KC, int FLUSH, bool KAHAN, int OCC>
+__global__ __launch_bounds__(256) void frame_kernel(const float* __restrict__ Lt, const float* __restrict__ Rt,
+                                                    float* __restrict__ O, int family, uint32_t rows,
+                                                    uint32_t cols, uint32_t n, uint32_t m, uint32_t nl_mats,
+                                                    uint32_t nr_mats, uint32_t pairs, uint32_t thresh,
+                                                    uint32_t lp_shift, frame_table tab) {
+    constexpr int G = kFrameGroup;
+    __shared__ uint32_t start[kMaxBorder + 1];
+    for (uint32_t i = threadIdx.x; i <= tab.nh; i += blockDim.x) start[i] = tab.start[i];
+    __syncthreads();
+    const int H = (int)rows, W = (int)cols, Wp = W + kFramePad;
+    const uint32_t per_quad = start[tab.nh];
+    const uint32_t slot = (blockIdx.x * blockDim.x + threadIdx.x) >> lp_shift;
+    if (slot >= 4 * per_quad) return;
+    const uint32_t q = slot / per_quad, t = slot - q * per_quad;
+    int lo = 0, hi = (int)tab.nh - 1;  // height index: largest with start[.] <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const int h = lo + 1;
+    const int cnt = min(W, (int)((thresh + h - 1) / h) - 1);
+    const int wlo = (int)(t - start[lo]) * G + 1;
+    int whi = min(cnt, wlo + G - 1);
+    const bool dyneg = q & 1, dxneg = q & 2;
+    if (dxneg) whi = min(whi, W - 1);     // dx = 0 belongs to the dx >= 0 quadrants
+    if ((dyneg && h == H) || wlo > whi) return;  // dy = 0 belongs to the dy >= 0 quadrants
+    const int rl = dyneg ? 0 : H - h, rr = dyneg ? H - h : 0;
+    const int ra = dxneg ? rr : rl, rb = dxneg ? rl : rr;
+    const int s0 = W - whi;  // output k: shift s0 + k, width whi - k
+    const uint32_t sa = dxneg ? nr_mats : nl_mats, sb = dxneg ? nl_mats : nr_mats;  // interleave strides
+    const int ng = (whi + G - 1) / G;  // column steps per row
+    const int nsteps = h * ng;
+    const int SY = 2 * H - 1, SX = 2 * W - 1;
+    const int dy = dyneg ? -(H - h) : H - h;
+    const uint32_t nchunks = ((pairs - 1) >> lp_shift) + 1;
+    for (uint32_t chunk = blockIdx.y; chunk < nchunks; chunk += gridDim.y) {
+        const uint32_t pk = (chunk << lp_shift) + (threadIdx.x & ((1u << lp_shift) - 1));
+        if (pk >= pairs) break;
+        uint32_t li, ri;
+        pair_of(family, n, m, pk, li, ri);
+        const float* __restrict__ A = (dxneg ? Rt : Lt) + (dxneg ? ri : li);
+        const float* __restrict__ B = (dxneg ? Lt : Rt) + (dxneg ? li : ri);
+        // element offsets (before the interleave stride) of step (i, g)
+        auto fetch = [&](int i, int g, float (&va)[2 * G], float (&vb)[G]) {
+            const float* __restrict__ ap = A + ((size_t)(ra + i) * Wp + s0 + G * g) * sa;
+            const float* __restrict__ bp = B + ((size_t)(rb + i) * Wp + G * g) * sb;
+#pragma unroll
+            for (int j = 0; j < 2 * G; ++j) va[j] = ap[(size_t)j * sa];
+#pragma unroll
+            for (int j = 0; j < G; ++j) vb[j] = bp[(size_t)j * sb];
+        };
+        double acc[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) acc[k] = 0.0;
+        float ca[2 * G], cb[G];
+        fetch(0, 0, ca, cb);
+        int i = 0, g = 0;
+        for (int st = 0; st < nsteps; ++st) {
+            if (++g == ng) {
+                g = 0;
+                ++i;
+            }
+            float na[2 * G], nb[G];
+            if (st + 1 < nsteps) fetch(i, g, na, nb);  // the next step, in flight during this one
+            double da[2 * G];
+#pragma unroll
+            for (int j = 0; j < 2 * G; ++j) da[j] = (double)ca[j];
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                const double bv = (double)cb[j];
+#pragma unroll
+                for (int k = 0; k < G; ++k) acc[k] = fma(da[j + k], bv, acc[k]);
+            }
+            if (st + 1 < nsteps) {
+#pragma unroll
+                for (int j = 0; j < 2 * G; ++j) ca[j] = na[j];
+#pragma unroll
+                for (int j = 0; j < G; ++j) cb[j] = nb[j];
+            }
+        }
+        float* __restrict__ op = O + (size_t)pk * SX * SY + (size_t)(dy + H - 1) * SX;
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int w = whi - k;
+            if (w >= wlo) op[(dxneg ? -(W - w) : W - w) + W - 1] = (float)acc[k];
+        }
+    }
+}
+
+template <int KIND, int WX, int WY, int KC, int FLUSH, bool KAHAN, int OCC, int JR = 1>
 void launch(const family_args& a, const float* L, const float* R, float* O, uint32_t skip_area, hipStream_t s) {
-    using G = geom<WX, WY, split_traits<KIND>::planes, KC>;
+    using G = geom<WX, WY, split_traits<KIND>::planes, KC, JR>;
     const uint32_t SX = 2 * a.cols - 1, SY = 2 * a.rows - 1;
     const uint32_t tiles_x = (SX + G::BX - 1) / G::BX;
     const uint32_t tiles_y = (SY + G::BY - 1) / G::BY;
     const uint64_t nblocks = (uint64_t)tiles_x * tiles_y * a.pairs;
     if (nblocks > 0x7fffffffULL) fail(XC_ERR_INVALID_ARGUMENT, "problem too large for one launch");
-    allow_dynamic_lds(reinterpret_cast<const void*>(&mfma_kernel<KIND, WX, WY, KC, FLUSH, KAHAN, OCC>), G::BYTES);
+    allow_dynamic_lds(reinterpret_cast<const void*>(&mfma_kernel<KIND, WX, WY, KC, FLUSH, KAHAN, OCC, JR>), G::BYTES);
     dim3 grid((uint32_t)nblocks), block(256);
     record_launch(KIND == XC_PREC_BF16X3   ? "mfma_kernel<bf16x3>"
                   : KIND == XC_PREC_FP16X3 ? "mfma_kernel<fp16x3>"
@@ -568,7 +687,7 @@ void launch(const family_args& a, const float* L, const float* R, float* O, uint
                                            : "mfma_kernel<fp16>",
                   grid, block, G::BYTES, a.rows, a.cols, a.pairs, 1);
     timed tm("mfma_kernel", s);
-    hipLaunchKernelGGL((mfma_kernel<KIND, WX, WY, KC, FLUSH, KAHAN, OCC>), grid, block, G::BYTES, s, L, R, O, a.family, a.rows, a.cols, a.n,
+    hipLaunchKernelGGL((mfma_kernel<KIND, WX, WY, KC, FLUSH, KAHAN, OCC, JR>), grid, block, G::BYTES, s, L, R, O, a.family, a.rows, a.cols, a.n,
                        a.m, tiles_x, tiles_y, skip_area);
     XC_HIP(hipGetLastError());
 }
@@ -578,22 +697,44 @@ void launch(const family_args& a, const float* L, const float* R, float* O, uint
 void launch_frame_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t thresh, hipStream_t s) {
     if (a.pairs == 0 || thresh <= 1) return;
     if (thresh > (uint32_t)kMaxBorder) fail(XC_ERR_INVALID_ARGUMENT, "border threshold above " + std::to_string(kMaxBorder));
+    // Tasks per quadrant: for each height h with frame outputs, ceil(widths / kFrameGroup).
     frame_table tab{};
     uint32_t total = 0, nh = 0;
     for (uint32_t h = 1; h <= a.rows && h < thresh; ++h) {
         const uint32_t cnt = std::min<uint32_t>(a.cols, (thresh + h - 1) / h - 1);
         if (cnt == 0) break;
         tab.start[nh++] = total;
-        total += cnt;
+        total += (cnt + kFrameGroup - 1) / kFrameGroup;
     }
     if (nh == 0) return;
     tab.nh = nh;
     tab.start[nh] = total;
-    const uint32_t blocks = (uint32_t)(((uint64_t)total * 4 + 3) / 4);  // one wave per (h, w, quadrant)
-    dim3 grid(blocks, std::min<uint32_t>(a.pairs, 65535));
+    // Padded interleaved copies of the left and right matrices.
+    const uint32_t nl = a.family == XC_ONE_TO_ONE || a.family == XC_ONE_TO_MANY ? 1 : a.n;
+    const uint32_t nr = a.family == XC_ONE_TO_ONE ? 1 : a.m;
+    const uint64_t padded = (uint64_t)a.rows * (a.cols + kFramePad);
+    float* next = static_cast<float*>(workspace(WS_FRAME, sizeof(float) * padded * (nl + nr), s));
+    auto interleave = [&](const float* X, uint32_t count) {
+        float* Xt = next;
+        next += padded * count;
+        const uint64_t gx = (padded + 31) / 32;
+        const uint32_t gy = (count + 31) / 32;
+        if (gx > 0x7fffffffULL || gy > 65535) fail(XC_ERR_INVALID_ARGUMENT, "border frame: matrices too large to interleave");
+        hipLaunchKernelGGL(interleave_kernel, dim3((uint32_t)gx, gy), dim3(256), 0, s, X, Xt, count, a.cols, padded);
+        XC_HIP(hipGetLastError());
+        return (const float*)Xt;
+    };
+    const float* Lt = interleave(L, nl);
+    const float* Rt = interleave(R, nr);
+    uint32_t lp_shift = 0;  // lanes per task: the power of two >= pairs, at most 64
+    while (lp_shift < 6 && (1u << lp_shift) < a.pairs) ++lp_shift;
+    const uint64_t lanes = (uint64_t)4 * total << lp_shift;
+    const uint64_t chunks = ((uint64_t)a.pairs + (1u << lp_shift) - 1) >> lp_shift;
+    if ((lanes + 255) / 256 > 0x7fffffffULL) fail(XC_ERR_INVALID_ARGUMENT, "border frame too large for one launch");
+    dim3 grid((uint32_t)((lanes + 255) / 256), (uint32_t)std::min<uint64_t>(chunks, 65535));
     timed tf("frame_kernel", s);
-    hipLaunchKernelGGL(frame_kernel, grid, dim3(256), 0, s, L, R, O, a.family, a.rows, a.cols, a.n, a.m, a.pairs,
-                       tab);
+    hipLaunchKernelGGL(frame_kernel, grid, dim3(256), 0, s, Lt, Rt, O, a.family, a.rows, a.cols, a.n, a.m, nl, nr,
+                       a.pairs, thresh, lp_shift, tab);
     XC_HIP(hipGetLastError());
 }
 
@@ -617,8 +758,6 @@ __global__ __launch_bounds__(256, kBOCC) void batch_kernel(const batch_item* __r
                                                           uint32_t thresh) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     __shared__ uint32_t s_item;
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     while (true) {
         if (threadIdx.x == 0) s_item = atomicAdd(&counters[0], 1u);
         __syncthreads();
@@ -638,16 +777,16 @@ __global__ __launch_bounds__(256, kBOCC) void batch_kernel(const batch_item* __r
         } else {
             const uint32_t* start = tables + pd.table_off;
             const uint32_t total = start[pd.nh] * 4;
-            for (uint32_t i = 0; i < kFrameChunkOutputs / 4; ++i) {
-                const uint32_t o = bi.b + wave * (kFrameChunkOutputs / 4) + i;
-                if (o >= total) break;
-                const uint32_t hw = o >> 2, q = o & 3;
+            static_assert(kFrameChunkOutputs == 256, "one frame output per thread of the block");
+            const uint32_t o = bi.b + threadIdx.x;
+            if (o < total) {  // quadrant-major, as frame_kernel
+                const uint32_t q = o / start[pd.nh], hw = o - q * start[pd.nh];
                 int h, w;
                 frame_hw(start, pd.nh, hw, h, w);
                 const int dy = (q & 1) ? -(H - h) : (H - h);
                 const int dx = (q & 2) ? -(W - w) : (W - w);
-                if (((q & 1) && dy == 0) || ((q & 2) && dx == 0)) continue;
-                frame_output(Lp, Rp, Op, H, W, dy, dx, h, w, lane);
+                if (!(((q & 1) && dy == 0) || ((q & 2) && dx == 0)))
+                    Op[(size_t)(dy + H - 1) * (2 * W - 1) + (dx + W - 1)] = frame_value(Lp, Rp, W, dy, dx, h, w);
             }
         }
     }
@@ -667,7 +806,7 @@ __global__ __launch_bounds__(256, kBOCC) void batch_kernel(const batch_item* __r
 uint32_t batch_tiles_x(uint32_t cols) { return (2 * cols - 1 + 64 * kBWX - 1) / (64 * kBWX); }
 uint32_t batch_tiles_y(uint32_t rows) { return (2 * rows - 1 + 64 * kBWY - 1) / (64 * kBWY); }
 uint32_t batch_frame_chunk() { return kFrameChunkOutputs; }
-uint32_t batch_default_border() { return kDefaultBorder; }
+uint32_t batch_default_border(int kind) { return kind == XC_PREC_FP16X3 ? kDefaultBorderFp16x3 : kDefaultBorder; }
 
 uint32_t frame_table_host(uint32_t rows, uint32_t cols, uint32_t thresh, std::vector<uint32_t>& start) {
     start.clear();
@@ -718,6 +857,8 @@ void launch_tile(const family_args& a, const float* L, const float* R, float* O,
     case 3: launch<KIND, 4, 1, 128, 2, true, 1>(a, L, R, O, skip_area, s); break;
     case 5: launch<KIND, 4, 1, 64, 8, false, 2>(a, L, R, O, skip_area, s); break;
     case 6: launch<KIND, 2, 2, 64, 4, false, 2>(a, L, R, O, skip_area, s); break;
+    case 7: launch<KIND, 4, 1, 128, 8, false, 2, 2>(a, L, R, O, skip_area, s); break;
+    case 8: launch<KIND, 4, 1, 128, 8, false, 1, 2>(a, L, R, O, skip_area, s); break;
     default: fail(XC_ERR_INVALID_ARGUMENT, "tile code " + std::to_string(tile) + " is not in the MFMA tile catalog");
     }
 }
@@ -726,7 +867,7 @@ void launch_tile(const family_args& a, const float* L, const float* R, float* O,
 
 void launch_mfma_f32(const family_args& a, const float* L, const float* R, float* O, int kind, uint32_t border,
                      uint32_t tile, hipStream_t s) {
-    if (tile > (uint32_t)mfma_catalog_size())
+    if (tile > 8)
         fail(XC_ERR_INVALID_ARGUMENT, "tile code " + std::to_string(tile) + " is not in the MFMA tile catalog");
     switch (kind) {
     case XC_PREC_BF16X3: {
@@ -736,7 +877,7 @@ void launch_mfma_f32(const family_args& a, const float* L, const float* R, float
         break;
     }
     case XC_PREC_FP16X3: {
-        const uint32_t thresh = border ? border : kDefaultBorder;
+        const uint32_t thresh = border ? border : kDefaultBorderFp16x3;
         launch_tile<XC_PREC_FP16X3>(a, L, R, O, tile, thresh, s);
         launch_frame_f32(a, L, R, O, thresh, s);
         break;

@@ -35,5 +35,7 @@ float time_candidate_f32(const family_args& a, const float* L, const float* R, f
                          uint32_t reps, hipStream_t s);
 void autotune_clear();
 xc_options auto_options_f32(const family_args& a, const float* L, const float* R, float* O, hipStream_t s);
+// XC_PREC_AUTO for double data: the FP64 direct kernel or the own FFT, timed once per workload.
+xc_options auto_options_f64(const family_args& a, const double* L, const double* R, double* O, hipStream_t s);
 
 }  // namespace xc

@@ -25,7 +25,7 @@ namespace xc {
 
 namespace {
 
-constexpr uint32_t kFeatures = 15;
+constexpr uint32_t kFeatures = 16;
 
 // ------------------------------------------------------------------ candidates
 std::string candidate_name(uint32_t i) {
@@ -221,7 +221,7 @@ void write_list(std::ostream& out, const char* key, const std::vector<double>& v
 
 }  // namespace
 
-uint32_t num_candidates() { return 3 + (uint32_t)mfma_catalog_size(); }
+uint32_t num_candidates() { return 4 + (uint32_t)mfma_catalog_size(); }
 
 xc_options candidate(uint32_t i) {
     xc_options o{};
@@ -231,9 +231,9 @@ xc_options candidate(uint32_t i) {
     } else if (i <= nt) {
         o.precision = XC_PREC_BF16X3;
         o.tile = i;
-    } else if (i == nt + 1 || i == nt + 2) {
+    } else if (i >= nt + 1 && i <= nt + 3) {
         o.precision = XC_PREC_FFT64;
-        o.tile = i - nt - 1;  // 0: rocFFT, 1: own FFT kernels
+        o.tile = i - nt - 1;  // 0: rocFFT, 1: own FFT kernels, 2: own FFT with power-of-two rows
     } else {
         fail(XC_ERR_INVALID_ARGUMENT, "candidate index out of range");
     }
@@ -241,11 +241,12 @@ xc_options candidate(uint32_t i) {
 }
 
 // Features: log2 of rows, cols, n, m, results and MACs; log2 of the FFT points transformed
-// at the reference's padding (2H x 2W); log2 of the power-of-two padding's area over the
-// needed (2H-1)(2W-1); log2 of the largest prime factor of 2H and of 2W (how well the
-// reference padding factors); the family one-hot; log2 of the own-FFT method's butterfly
-// work (row transforms of every input, two column transforms and one row transform per
-// result, each n log2 n at the power-of-two sizes).
+// at the reference's padding (2H x 2W); log2 of the own FFT's padded area (fftk_col_len x
+// fftk_row_len) over the needed (2H-1)(2W-1); log2 of the largest prime factor of 2H and of
+// 2W (how well the reference padding factors); the family one-hot; log2 of the own-FFT
+// method's butterfly work (row transforms of every input, two column transforms and one row
+// transform per result, each n log2 n at its lengths); log2 of the power-of-two row length
+// over the own FFT's row length (0, or log2(4/3) where rows of 3 * 2^n are shorter).
 uint32_t workload_features(const xc_workload& w, float* f, uint32_t cap) {
     const int64_t k = num_results((int)w.family, w.n, w.m);
     if (w.family > XC_N_TO_M || k < 0 || w.rows == 0 || w.cols == 0)
@@ -269,12 +270,7 @@ uint32_t workload_features(const xc_workload& w, float* f, uint32_t cap) {
             }
         return x > 1 ? std::max(best, x) : best;
     };
-    auto pow2ceil = [](uint32_t x) {
-        uint32_t p = 1;
-        while (p < x) p <<= 1;
-        return p;
-    };
-    const double pr = std::max(16u, pow2ceil(2 * w.rows - 1)), pc = std::max(16u, pow2ceil(2 * w.cols - 1));
+    const double pr = fftk_col_len(w.rows), pc = fftk_row_len(w.cols), pc2 = fftk_col_len(w.cols);
     f[0] = (float)lr;
     f[1] = (float)lc;
     f[2] = (float)std::log2((double)w.n);
@@ -290,6 +286,7 @@ uint32_t workload_features(const xc_workload& w, float* f, uint32_t cap) {
     const double own = nin * (double)w.rows * pc * std::log2(pc) / 2 + (double)k * (pc / 2 + 1) * pr * std::log2(pr) * 2 +
                        (double)k * pr * pc * std::log2(pc) / 2;
     f[14] = (float)std::log2(own);
+    f[15] = (float)std::log2(pc2 / pc);
     return kFeatures;
 }
 
@@ -433,6 +430,60 @@ xc_options auto_options_f32(const family_args& a, const float* L, const float* R
         return c;
     }
     return candidate(autotune_f32(a, L, R, O, 2, nullptr, s));
+}
+
+// Double data: the FP64 direct kernel (default tile) or the own FFT (FP64 throughout), timed
+// once per workload and device on the call's buffers; the own FFT alone where the direct
+// kernel would take seconds (above 1e12 multiply-adds).
+xc_options auto_options_f64(const family_args& a, const double* L, const double* R, double* O, hipStream_t s) {
+    static std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, int>, int> tuned;
+    int dev = 0;
+    XC_HIP(hipGetDevice(&dev));
+    const auto key = std::make_tuple((uint32_t)a.family, a.rows, a.cols, a.n, a.m, dev);
+    xc_options direct{}, fft{};
+    direct.precision = XC_PREC_FP32;  // for double data: the FP64 direct kernel
+    fft.precision = XC_PREC_FFT64;
+    fft.tile = 1;
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        auto it = tuned.find(key);
+        if (it != tuned.end()) return it->second ? fft : direct;
+    }
+    const bool fft_ok = a.rows <= 2048 && a.cols <= 2048;
+    const double macs = (double)a.pairs * a.rows * a.rows * a.cols * a.cols;
+    int pick = fft_ok ? 1 : 0;
+    if (fft_ok && macs <= 1e12) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        XC_HIP(hipEventCreate(&e0));
+        XC_HIP(hipEventCreate(&e1));
+        struct ev_guard {
+            hipEvent_t a, b;
+            ~ev_guard() {
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+            }
+        } eg{e0, e1};
+        auto time = [&](bool use_fft) {
+            float best = INFINITY;
+            for (int rep = 0; rep < 2; ++rep) {  // the first run pays workspace and attribute set-up
+                XC_HIP(hipEventRecord(e0, s));
+                if (use_fft)
+                    run_fftk_f64(a, L, R, O, 0, false, s);
+                else
+                    launch_direct_f64(a, L, R, O, direct, s);
+                XC_HIP(hipEventRecord(e1, s));
+                XC_HIP(hipEventSynchronize(e1));
+                float t = 0.f;
+                XC_HIP(hipEventElapsedTime(&t, e0, e1));
+                best = std::min(best, t);
+            }
+            return best;
+        };
+        pick = time(true) <= time(false) ? 1 : 0;
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    tuned[key] = pick;
+    return pick ? fft : direct;
 }
 
 void cost_model_fit(const char* db_path, const char* model_path, const xc_train_options& o, uint32_t holdout_mod) {

@@ -65,7 +65,8 @@ typedef enum xc_family {
  *  XC_PREC_BF16     MFMA, single bf16 product (fast, ~1e-3 relative error)
  *  XC_PREC_FP16     MFMA, single fp16 product (inputs must fit fp16 range)
  *  XC_PREC_FP16X3   MFMA, each operand split into fp16 hi + lo (hi*hi + hi*lo + lo*hi in
- *                   FP32, border outputs exact in FP32 as for bf16x3); 22 significant bits
+ *                   FP32, border outputs exact in FP32 as for bf16x3, default threshold
+ *                   16 where bf16x3's is 512); 22 significant bits
  *                   per operand, inputs must fit fp16 range (|x| < 65504; the lo part of
  *                   |x| < 2^-3 loses bits to fp16 subnormals)
  *  XC_PREC_FFT64    FFT method computed in FP64, results in FP32; tile 0: rocFFT D2Z/Z2D,
@@ -90,7 +91,8 @@ typedef struct xc_options {
     uint32_t split;       /* FP32: split of the reduction (overlap rows) across blocks;
                              FFT64 tile 1: at most this many pairs per launch of the column
                              and inverse-row kernels (less scratch memory); 0 = auto */
-    uint32_t border;      /* BF16X3: overlap-area threshold for exact FP32 recompute, 0 = default */
+    uint32_t border;      /* BF16X3/FP16X3: overlap-area threshold (1..512) for exact FP32
+                             recompute; 0 = default (512 for bf16x3, 16 for fp16x3) */
     /* Register tile and schedule of the direct kernel (XC_PREC_FP32, and double data's
      * direct method); 0 = the library's default for the family and size.  Each value is
      * rounded down to a power of two and the tile to the largest compiled one not above the

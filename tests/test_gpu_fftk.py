@@ -205,4 +205,29 @@ def test_size_limit_is_rejected_before_device_work():
         x.cross_corr("one_to_one", left, right, precision="fft64", tile=1)
     assert e.value.status == 4
     with pytest.raises(x.XcError):
-        x.cross_corr("one_to_one", left[:, :8], right[:, :8], precision="fft64", tile=2)
+        x.cross_corr("one_to_one", left[:, :8], right[:, :8], precision="fft64", tile=3)
+
+
+@pytest.mark.parametrize("family,rows,cols,n,m", [("one_to_many", 37, 40, 1, 4), ("n_to_m", 50, 150, 2, 3),
+                                                  ("one_to_one", 140, 270, 1, 1), ("n_to_mn", 9, 600, 2, 4)])
+def test_power_of_two_rows_variant(family, rows, cols, n, m):
+    # tile 2 transforms rows at the power of two >= 2W-1 where tile 1 takes 3 * 2^n: the same
+    # correlation through other transform lengths, for float and double data.
+    for dtype, tol in ((np.float32, 1e-6), (np.float64, 1e-9)):
+        left = seeded((n, rows, cols), "u11", 61, dtype)
+        right = seeded((m, rows, cols), "u11", 62, dtype)
+        got = run(family, left, right, tile=2)
+        check(family, left, right, got, tol)
+
+
+def test_double_data_auto_picks_the_own_fft_at_c3():
+    # AUTO for float64 data times the FP64 direct kernel and the own FFT once per shape; at
+    # 16 x 64 pairs of 64x64 the FFT is more than 10x faster.
+    left = seeded((16, 64, 64), "normal", 3, np.float64)
+    right = seeded((64, 64, 64), "normal", 4, np.float64)
+    x = xcorr()
+    for _ in range(2):
+        out = x.cross_corr("n_to_m", dev(left), dev(right), precision="auto")
+        torch.cuda.synchronize()
+        assert x.last_launch()["kernel"].startswith("fftk"), x.last_launch()["kernel"]
+    check("n_to_m", left, right, out.cpu().numpy(), 1e-9)

@@ -135,3 +135,42 @@ def test_linearity_in_the_right_matrix():
     den = oracle_family("one_to_many", left.astype(np.float64), np.abs(both).astype(np.float64), absval=True)
     lin = 0.5 * got[0] + 0.25 * got[1]
     assert (np.abs(got[2] - lin) / den[2]).max() <= 4e-6
+
+
+@pytest.mark.parametrize("tile", range(0, 9))
+@pytest.mark.parametrize("family,rows,cols,n,m,dist", [CASES[2], CASES[7], CASES[10], CASES[13]])
+def test_every_mfma_tile(tile, family, rows, cols, n, m, dist):
+    """Tile codes 0-6 are the catalog the cost model chooses from; 7 and 8 stage two left rows
+    per block barrier (a larger ring and left double buffer) and are selectable by code."""
+    left = seeded((n, rows, cols), dist, 1500 + rows, np.float32)
+    right = seeded((m, rows, cols), dist, 1600 + cols, np.float32)
+    err, _, _ = errors(family, left, right, run(family, left, right, precision="bf16x3", tile=tile))
+    assert err.max() <= 2e-6, err.max()
+
+
+def test_mfma_tile_code_out_of_catalog_is_refused():
+    left = seeded((1, 16, 16), "u01", 1, np.float32)
+    with pytest.raises(Exception, match="catalog"):
+        run("one_to_one", left, left, precision="bf16x3", tile=9)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])
+@pytest.mark.parametrize("family,rows,cols,n,m,border", [
+    ("one_to_one", 40, 70, 1, 1, 512), ("one_to_many", 30, 17, 1, 3, 100), ("n_to_mn", 25, 33, 3, 6, 300),
+    ("n_to_m", 20, 24, 9, 8, 512), ("n_to_m", 64, 9, 5, 27, 200), ("one_to_many", 300, 5, 1, 2, 512)])
+def test_border_frame_outputs_are_rounded_once(prec, family, rows, cols, n, m, border):
+    """Outputs with overlap area below the border threshold are the exact sum rounded once to
+    FP32: within half an FP32 ulp of the FP64 oracle.  Pair counts of 1 to 135 cover the task
+    grid's lanes-per-task choices (1 to 64 pairs per task, a partial last chunk)."""
+    left = seeded((n, rows, cols), "u11", 31 + rows, np.float32)
+    right = seeded((m, rows, cols), "u11", 37 + cols, np.float32)
+    got = run(family, left, right, precision=prec, border=border).astype(np.float64)
+    _, ref, den = errors(family, left, right, got)
+    dy = rows - np.abs(np.arange(-(rows - 1), rows))[:, None]
+    dx = cols - np.abs(np.arange(-(cols - 1), cols))[None, :]
+    frame = (dy * dx) < border
+    # half an FP32 ulp of the value, plus the FP64 summation slack of both sums (area * 2^-53
+    # of the sum of magnitudes each), which matters only where the sum cancels
+    tol = np.abs(ref[:, frame]) * 2.0 ** -24 * 1.0001 + (dy * dx)[frame] * 2.0 ** -52 * den[:, frame]
+    assert np.all(np.abs(got[:, frame] - ref[:, frame]) <= tol + 1e-45)
+    assert (np.abs(got - ref) / np.where(den > 0, den, 1.0)).max() <= 2e-6

@@ -33,13 +33,16 @@ def py_features(family, rows, cols, n, m):
     def p2(v):
         return max(16, 1 << int(np.ceil(np.log2(v))))
 
-    pr, pc = p2(2 * rows - 1), p2(2 * cols - 1)
+    def row_len(c):  # the power of two, or 3 * 2^e (e = 4..9) when shorter
+        return min([p2(2 * c - 1)] + [3 * 2 ** e for e in range(4, 10) if 3 * 2 ** e >= 2 * c - 1])
+
+    pr, pc, pc2 = p2(2 * rows - 1), row_len(cols), p2(2 * cols - 1)
     pad = np.log2(pr * pc / ((2 * rows - 1) * (2 * cols - 1)))
     nin = {0: 2, 1: 1 + m, 2: n + m, 3: n + m}[family]
     own = nin * rows * pc * np.log2(pc) / 2 + k * (pc / 2 + 1) * pr * np.log2(pr) * 2 + k * pr * pc * np.log2(pc) / 2
     return [lr, lc, np.log2(n), np.log2(m), np.log2(k), np.log2(k) + 2 * lr + 2 * lc,
             np.log2(transforms) + 2 + lr + lc, pad, np.log2(lpf(2 * rows)), np.log2(lpf(2 * cols))] + \
-        [float(family == i) for i in range(4)] + [np.log2(own)]
+        [float(family == i) for i in range(4)] + [np.log2(own), np.log2(pc2 / pc)]
 
 
 def parse_model(path):
@@ -75,7 +78,7 @@ def x():
 
 def test_features_match_definition(x):
     for fam, r, c, n, m in [(0, 16, 16, 1, 1), (1, 256, 256, 1, 64), (2, 100, 37, 3, 12), (3, 64, 64, 16, 64),
-                            (0, 1, 1, 1, 1), (0, 177, 1024, 1, 1)]:
+                            (0, 1, 1, 1, 1), (0, 177, 1024, 1, 1), (1, 300, 300, 1, 5), (3, 40, 150, 2, 3)]:
         np.testing.assert_allclose(x.features(FAM[fam], r, c, n, m), py_features(fam, r, c, n, m), rtol=1e-6)
 
 
