@@ -278,6 +278,8 @@ def test_every_algorithm_validates_against_fixture(name, dtype, tmp_path):
     tol = 2e-6 if dtype == "single" else 1e-12
     if name.startswith("fft_") and dtype == "single":
         tol = 1e-3  # FP32 FFT: absolute error spread over all outputs
+    if name.startswith("auto") and dtype == "double":
+        tol = 1e-9  # AUTO may run the own FFT in FP64 (rounding global, ~1e-16 of the largest output)
     assert norm_error(got, ref, den)[0] < tol, name
 
 

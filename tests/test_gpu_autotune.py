@@ -43,7 +43,8 @@ def test_autotune_times_every_candidate_and_auto_uses_the_winner():
 
 def test_autotune_beyond_1e13_macs_runs_only_the_fft_candidates():
     """10 pairs of 1024^2 (1.1e13 MACs): the definition-based candidates (about a second
-    each per run here) are not run; the own FFT wins."""
+    each per run here) are not run; the own FFT wins (tile 1 or 2: at 1024 columns both take
+    2048-point rows, so the two run the same kernels and either may time faster)."""
     x = xcorr()
     x.autotune_clear()
     L = torch.rand((1, 1024, 1024), device="cuda")
@@ -56,7 +57,7 @@ def test_autotune_beyond_1e13_macs_runs_only_the_fft_candidates():
             assert math.isfinite(t), (name, t)
         else:
             assert t == float("inf"), (name, t)
-    assert cands[best][1:] == (x.PREC_FFT64, 1)
+    assert cands[best][1] == x.PREC_FFT64 and cands[best][2] in (1, 2)
     # sum rule on the tuned AUTO result: sum of a correlation = sum(L) * sum(R)
     got = x.cross_corr("one_to_many", L, R, out=out, precision="auto")
     s = got.double().sum(dim=(1, 2))
