@@ -21,7 +21,8 @@ void run_family_f32(const family_args& a, const float* L, const float* R, float*
 // Split-precision MFMA implicit GEMM (xc_mfma.hip); `kind` is an xc_precision value.
 void launch_mfma_f32(const family_args& a, const float* L, const float* R, float* O, int kind,
                      uint32_t border, uint32_t tile, hipStream_t s);
-int mfma_catalog_size();
+int mfma_catalog_size();          // MFMA tiles among the cost model's candidates
+uint32_t mfma_candidate_tile(int i);  // tile code of MFMA candidate i
 
 // Uneven batch: work list of the persistent kernel (xc_mfma.hip) and its host plan (xc_batch.cpp).
 struct batch_pair_desc {

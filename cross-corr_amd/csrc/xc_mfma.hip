@@ -502,6 +502,320 @@ __global__ __launch_bounds__(256, OCC) void mfma_kernel(const float* __restrict_
                                                    skip_area, smem);
 }
 
+// Row-split block tile: the block's 4 waves compute the same 64 dy x 32*NS dx outputs (2 x NS
+// sub-tiles of 32 x 32 each), each wave over its own left rows (wave w takes rows r_lo + w,
+// r_lo + w + 4, ...).  Every wave then has the same column ranges and nearly the same work per
+// step (in mfma_tile the waves own different dx and their column ranges differ, so the faster
+// ones wait at each barrier).  The waves' totals are added through LDS at the end, each of the
+// first 2*NS waves finishing one sub-tile.  Staging as mfma_tile with four left rows per step.
+template <int NS, int PLANES, int KC>
+struct rgeom {
+    static constexpr int NW = 4;                          // waves = left rows per step
+    static constexpr int BX = 32 * NS, BY = 64;
+    static constexpr int NSLOT = BY + 2 * NW - 1;
+    static constexpr int RS = KC * 2 + 16;
+    static constexpr int LENA = KC + BX + 64;
+    static constexpr int CS = ((LENA * 2 + 127) / 128) * 128 + 32;
+    static constexpr int SS = ((PLANES * RS / 16) % 2) ? PLANES * RS : PLANES * RS + 16;
+    static constexpr int RING = NSLOT * SS;
+    static constexpr int LBUF = PLANES * 4 * CS;
+    static constexpr int STAGE = RING + 2 * NW * LBUF;
+    static constexpr int RED = NW * 2 * NS * 16 * 64 * 4;  // the waves' totals, for the final sum
+    static constexpr int BYTES = STAGE > RED ? STAGE : RED;
+};
+
+// MFMAs of one left-row step of a one-sub-tile-wide (NS = 1) row-split wave over chunks
+// [q0, q1): UM bit u selects dy sub-tile u.
+template <int KIND, int P, int UM>
+__device__ __forceinline__ void chunk_range1(f32x16 (&acc)[2][1], const char* const (&arow)[2], const char* bcol,
+                                             int q0, int q1, int rs, int cs) {
+    for (int qq = q0; qq < q1; qq += 16) {
+        uint4 a[2][P], b[P];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int pl = 0; pl < P; ++pl)
+                if (UM & (1 << u)) a[u][pl] = *reinterpret_cast<const uint4*>(arow[u] + pl * rs + qq * 2);
+#pragma unroll
+        for (int pl = 0; pl < P; ++pl) {
+            const uint2* q = reinterpret_cast<const uint2*>(bcol + pl * 4 * cs + qq * 2);
+            const uint2 v0 = q[0], v1 = q[1];
+            b[pl] = make_uint4(v0.x, v0.y, v1.x, v1.y);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (UM & (1 << u)) acc[u][0] = mma<KIND>(a[u][0], b[0], acc[u][0]);
+        if constexpr (P == 2) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+                if (UM & (1 << u)) acc[u][0] = mma<KIND>(a[u][0], b[1], acc[u][0]);
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+                if (UM & (1 << u)) acc[u][0] = mma<KIND>(a[u][1], b[0], acc[u][0]);
+        }
+    }
+}
+
+template <int KIND, int NS, int KC, int FLUSH, int OCC, bool PIPE_ROWS = (OCC == 1)>
+__device__ __forceinline__ void mfma_rows_tile(const float* __restrict__ Lp, const float* __restrict__ Rp,
+                                               float* __restrict__ Op, const int H, const int W, const uint32_t tx,
+                                               const uint32_t ty, const uint32_t skip_area, char* smem) {
+    using S = split_traits<KIND>;
+    constexpr int P = S::planes;
+    using G = rgeom<NS, P, KC>;
+    constexpr int NW = G::NW;
+    constexpr bool PIPE = PIPE_ROWS;
+    char* ring = smem;
+    char* lbuf = smem + G::RING;
+    const int SX = 2 * W - 1;
+    const int dxb = (int)tx * G::BX - (W - 1);
+    const int dyb = (int)ty * G::BY - (H - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int li32 = lane & 31, g = lane >> 5;
+    const int dxs[2] = {dxb, dxb + 32};  // sub-tile 1 only where NS == 2
+    const int dyw = dyb;
+
+    const int dy_hi_b = min(dyb + G::BY - 1, H - 1);
+    const int r_lo = max(0, dyb);
+    const int r_hi = min(H, H + dy_hi_b);
+    const int dx_hi_b = min(dxb + G::BX - 1, W - 1);
+    const int cb_lo = max(0, -dx_hi_b), cb_hi = min(W, W - dxb);
+
+    f32x16 tot[2][NS], acc[2][NS];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) tot[u][s][e] = acc[u][s][e] = 0.f;
+
+    static_assert(G::LENA % 4 == 0 && KC % 4 == 0, "quad staging");
+    constexpr int LQ = G::LENA / 4;
+    constexpr int LITEMS = 4 * LQ;
+    constexpr int LPT = (LITEMS + 255) / 256;
+    constexpr int RQ = KC / 4;
+    static_assert(NW * RQ <= 256, "one thread per staged right-row quad");
+    auto load_left = [&](int r, int cbase, int item) -> float4 {
+        const int t = item / LQ, j = item - (item / LQ) * LQ;
+        const int c = cbase + 4 * j + t;
+        float v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            v[k] = ((unsigned)r < (unsigned)H && (unsigned)(c + k) < (unsigned)W) ? Lp[(size_t)r * W + c + k] : 0.f;
+        return make_float4(v[0], v[1], v[2], v[3]);
+    };
+    auto load_right = [&](int row, int c0, int j) -> float4 {
+        const int c = c0 + 4 * j;
+        float v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            v[k] = ((unsigned)row < (unsigned)H && c + k < W) ? Rp[(size_t)row * W + c + k] : 0.f;
+        return make_float4(v[0], v[1], v[2], v[3]);
+    };
+    auto split4 = [&](float4 v, uint2& hi, uint2& lo) {
+        uint16_t h[4], l[4];
+        S::split(v.x, h[0], l[0]);
+        S::split(v.y, h[1], l[1]);
+        S::split(v.z, h[2], l[2]);
+        S::split(v.w, h[3], l[3]);
+        hi = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
+        lo = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
+    };
+    auto put_left = [&](int buf, int item, float4 v) {
+        const int t = item / LQ, j = item - (item / LQ) * LQ;
+        uint2 hi, lo;
+        split4(v, hi, lo);
+        char* base = lbuf + buf * G::LBUF + t * G::CS + j * 8;
+        *reinterpret_cast<uint2*>(base) = hi;
+        if (P == 2) *reinterpret_cast<uint2*>(base + 4 * G::CS) = lo;
+    };
+    auto put_right = [&](int row, int j, float4 v) {
+        uint2 hi, lo;
+        split4(v, hi, lo);
+        char* base = ring + ring_slot(row, G::NSLOT) * G::SS + j * 8;
+        *reinterpret_cast<uint2*>(base) = hi;
+        if (P == 2) *reinterpret_cast<uint2*>(base + G::RS) = lo;
+    };
+
+    // The two dx sub-tiles' column ranges (all waves alike), as in mfma_tile.
+    int cw_lo = W, cw_hi = 0;
+#pragma unroll
+    for (int sidx = 0; sidx < NS; ++sidx) {
+        const int lo = dxs[sidx], hi = min(lo + 31, W - 1);
+        if (lo <= hi) {
+            cw_lo = min(cw_lo, max(0, -hi));
+            cw_hi = max(cw_hi, min(W, W - lo));
+        }
+    }
+    for (int c0 = (cb_lo / KC) * KC; c0 < cb_hi; c0 += KC) {
+        const int cbase = c0 + dxb;
+        // Prefill: right rows r - dy for r in [r_lo, r_lo + NW), dy in the block; left rows
+        // r_lo .. r_lo + NW - 1 into buffers 0 .. NW - 1.
+        for (int idx = threadIdx.x; idx < (G::BY + NW - 1) * RQ; idx += 256) {
+            const int ro = idx / RQ, j = idx - (idx / RQ) * RQ;
+            const int row = r_lo + NW - 1 - dyb - ro;
+            put_right(row, j, load_right(row, c0, j));
+        }
+        for (int i = 0; i < NW; ++i)
+            for (int item = threadIdx.x; item < LITEMS; item += 256) put_left(i, item, load_left(r_lo + i, cbase, item));
+        __syncthreads();
+
+        const int q_lo = max(cw_lo, c0), q_hi = min(cw_hi, c0 + KC);
+        int ks_lo[2] = {0, 0}, ks_hi[2] = {0, 0};
+#pragma unroll
+        for (int sidx = 0; sidx < NS; ++sidx) {
+            const int lo = dxs[sidx], hi = min(lo + 31, W - 1);
+            const int a0 = max(max(0, -hi), c0) - c0, a1 = min(min(W, W - lo), c0 + KC) - c0;
+            if (lo <= hi && a0 < a1) {
+                ks_lo[sidx] = a0 & ~15;
+                ks_hi[sidx] = (a1 + 15) & ~15;
+            } else {
+                ks_lo[sidx] = ks_hi[sidx] = 0;
+            }
+        }
+
+        int step = 0;
+        for (int r0 = r_lo; r0 < r_hi; r0 += NW, ++step) {
+            const int cur = step & 1;
+            const bool more = r0 + NW < r_hi;
+            float4 pl[NW][LPT];
+            float4 pr = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int i = 0; i < NW; ++i)
+#pragma unroll
+                for (int k = 0; k < LPT; ++k) {
+                    const int item = threadIdx.x + k * 256;
+                    pl[i][k] = (more && item < LITEMS && r0 + NW + i < r_hi) ? load_left(r0 + NW + i, cbase, item)
+                                                                            : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+            if (more && threadIdx.x < NW * RQ)
+                pr = load_right(r0 + NW + (int)threadIdx.x / RQ - dyb, c0, (int)threadIdx.x % RQ);
+
+            const int r = r0 + wave;  // this wave's left row
+            if (r < r_hi) {
+                bool act_u[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int lo = dyw + 32 * u, hi = min(lo + 31, H - 1);
+                    act_u[u] = lo <= hi && r >= max(0, lo) && r < min(H, H + hi);
+                }
+                if (q_lo < q_hi && (act_u[0] || act_u[1])) {
+                    const char* lb = lbuf + (cur * NW + wave) * G::LBUF;
+                    const char* arow[2];
+#pragma unroll
+                    for (int u = 0; u < 2; ++u)
+                        arow[u] = ring + ring_slot(r - (dyw + 32 * u + li32), G::NSLOT) * G::SS + g * 16;
+                    const char* bcol[2];
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        const int p = 8 * g + (dxs[s] - dxb) + li32;
+                        const int t = p & 3;
+                        bcol[s] = lb + t * G::CS + (p - t) * 2;
+                    }
+                    if constexpr (NS == 1) {
+                        const int q0 = ks_lo[0], q1 = ks_hi[0];
+                        if (q0 < q1) {
+                            if (act_u[0] && act_u[1]) chunk_range1<KIND, P, 3>(acc, arow, bcol[0], q0, q1, G::RS, G::CS);
+                            else if (act_u[0]) chunk_range1<KIND, P, 1>(acc, arow, bcol[0], q0, q1, G::RS, G::CS);
+                            else chunk_range1<KIND, P, 2>(acc, arow, bcol[0], q0, q1, G::RS, G::CS);
+                        }
+                    } else {
+                    const int umask = (act_u[0] ? 3 : 0) | (act_u[1] ? 12 : 0);
+                    const int k0lo = ks_lo[0], k0hi = ks_hi[0], k1lo = ks_lo[1], k1hi = ks_hi[1];
+                    const bool e0 = k0lo >= k0hi;
+                    for (int seg = 0; seg < 3; ++seg) {
+                        int q0, q1, mask;
+                        if (seg == 0) {
+                            q0 = k1lo;
+                            q1 = e0 ? k1hi : min(k1hi, k0lo);
+                            mask = umask & 10;
+                        } else if (seg == 1) {
+                            q0 = e0 ? 0 : max(k0lo, k1lo);
+                            q1 = e0 ? 0 : min(k0hi, k1hi);
+                            mask = umask;
+                        } else {
+                            q0 = e0 ? 0 : max(k0lo, k1hi);
+                            q1 = e0 ? 0 : k0hi;
+                            mask = umask & 5;
+                        }
+                        if (q0 < q1 && mask)
+                            chunk_range_dispatch<KIND, P, PIPE>(mask, acc, arow, bcol, q0, q1, G::RS, G::CS);
+                    }
+                    }
+                }
+            }
+            if (((step + 1) % FLUSH) == 0 || !more) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u)
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        tot[u][s] += acc[u][s];
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) acc[u][s][e] = 0.f;
+                    }
+            }
+            if (more) {
+#pragma unroll
+                for (int i = 0; i < NW; ++i)
+#pragma unroll
+                    for (int k = 0; k < LPT; ++k) {
+                        const int item = threadIdx.x + k * 256;
+                        if (item < LITEMS) put_left((cur ^ 1) * NW + i, item, pl[i][k]);
+                    }
+                if (threadIdx.x < NW * RQ)
+                    put_right(r0 + NW + (int)threadIdx.x / RQ - dyb, (int)threadIdx.x % RQ, pr);
+            }
+            __syncthreads();
+        }
+    }
+
+    // Sum of the waves' totals: each wave writes its 2*NS sub-tiles, then wave w < 2*NS
+    // finishes sub-tile w.
+    constexpr int NT = 2 * NS;
+    float* red = reinterpret_cast<float*>(smem);  // [wave][sub-tile][element][lane]
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) red[((wave * NT + NS * u + s) * 16 + e) * 64 + lane] = tot[u][s][e];
+    __syncthreads();
+    if (wave >= NT) return;
+    const int u = wave / NS, s = wave % NS;
+    const int dx = dxs[s] + li32;
+    if (dx > W - 1) return;
+#pragma unroll
+    for (int a = 0; a < 16; ++a) {
+        float v = 0.f;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) v += red[((w * NT + wave) * 16 + a) * 64 + lane];
+        // D layout of v_mfma_f32_32x32x16: lane -> column (dx), register a -> row
+        // 8*(a/4) + 4*(lane/32) + a%4 (dy).
+        const int dy = dyw + 32 * u + 8 * (a >> 2) + 4 * g + (a & 3);
+        if (dy <= H - 1 && (uint32_t)(H - abs(dy)) * (uint32_t)(W - abs(dx)) >= skip_area)
+            Op[(size_t)(dy + H - 1) * SX + (dx + W - 1)] = v;
+    }
+}
+
+template <int KIND, int NS, int KC, int FLUSH, int OCC, bool PIPE_ROWS>
+__global__ __launch_bounds__(256, OCC) void mfma_rows_kernel(const float* __restrict__ L, const float* __restrict__ R,
+                                                           float* __restrict__ O, int family, uint32_t rows,
+                                                           uint32_t cols, uint32_t n, uint32_t m, uint32_t tiles_x,
+                                                           uint32_t tiles_y, uint32_t skip_area) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int H = (int)rows, W = (int)cols;
+    uint32_t bid = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tx = bid % tiles_x;
+    bid /= tiles_x;
+    const uint32_t ty = bid % tiles_y;
+    const uint32_t k = bid / tiles_y;
+    uint32_t li, ri;
+    pair_of(family, n, m, k, li, ri);
+    mfma_rows_tile<KIND, NS, KC, FLUSH, OCC, PIPE_ROWS>(L + (size_t)li * H * W, R + (size_t)ri * H * W,
+                                         O + (size_t)k * (2 * W - 1) * (2 * H - 1), H, W, tx, ty, skip_area, smem);
+}
+
 // Frame outputs of a (rows, cols, threshold): the overlap sizes (h, w) with h*w < threshold,
 // enumerated h-major; start[h-1] is the index of (h, 1) and start[nh] the total count.
 struct frame_table {
@@ -692,6 +1006,27 @@ void launch(const family_args& a, const float* L, const float* R, float* O, uint
     XC_HIP(hipGetLastError());
 }
 
+template <int KIND, int NS, int KC, int FLUSH, int OCC, bool PIPE_ROWS = (OCC == 1)>
+void launch_rows(const family_args& a, const float* L, const float* R, float* O, uint32_t skip_area, hipStream_t s) {
+    using G = rgeom<NS, split_traits<KIND>::planes, KC>;
+    const uint32_t SX = 2 * a.cols - 1, SY = 2 * a.rows - 1;
+    const uint32_t tiles_x = (SX + G::BX - 1) / G::BX;
+    const uint32_t tiles_y = (SY + G::BY - 1) / G::BY;
+    const uint64_t nblocks = (uint64_t)tiles_x * tiles_y * a.pairs;
+    if (nblocks > 0x7fffffffULL) fail(XC_ERR_INVALID_ARGUMENT, "problem too large for one launch");
+    allow_dynamic_lds(reinterpret_cast<const void*>(&mfma_rows_kernel<KIND, NS, KC, FLUSH, OCC, PIPE_ROWS>), G::BYTES);
+    dim3 grid((uint32_t)nblocks), block(256);
+    record_launch(KIND == XC_PREC_BF16X3   ? "mfma_rows_kernel<bf16x3>"
+                  : KIND == XC_PREC_FP16X3 ? "mfma_rows_kernel<fp16x3>"
+                  : KIND == XC_PREC_BF16   ? "mfma_rows_kernel<bf16>"
+                                           : "mfma_rows_kernel<fp16>",
+                  grid, block, G::BYTES, a.rows, a.cols, a.pairs, 1);
+    timed tm("mfma_kernel", s);
+    hipLaunchKernelGGL((mfma_rows_kernel<KIND, NS, KC, FLUSH, OCC, PIPE_ROWS>), grid, block, G::BYTES, s, L, R, O, a.family, a.rows,
+                       a.cols, a.n, a.m, tiles_x, tiles_y, skip_area);
+    XC_HIP(hipGetLastError());
+}
+
 }  // namespace
 
 void launch_frame_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t thresh, hipStream_t s) {
@@ -840,25 +1175,39 @@ uint32_t batch_lds_bytes(int kind) {
 uint32_t batch_blocks_per_cu() { return kBOCC; }
 
 // Tile catalog of the split-precision kernel (the runtime replacement of the reference's
-// compile-time template limits, CMakeLists.txt:16-45): xc_options.tile selects an entry,
-// 0 is the default.
-int mfma_catalog_size() { return 6; }
+// compile-time template limits, CMakeLists.txt:16-45): xc_options.tile selects an entry, 0 the
+// default for the size.  Codes 1-8 are the dx-split block tiles (mfma_tile: WX x WY waves own
+// different outputs; KC columns per pass, FLUSH, Kahan, blocks per CU, left rows per step),
+// 9-12 the row-split tiles (mfma_rows_tile: the waves share the outputs and split the rows).
+// Measured on one MI355X (bf16x3 without the frame): 64 x 256^2 one_to_many, tile 9 2.15 ms,
+// 11 2.17, 10 2.68, 12 2.76, 4 2.79, 6 3.96; 16 x 64 of 64^2 n_to_m, tile 6 0.27 ms, 10 0.37,
+// 9 0.39, 4 0.39, 12 0.43.  The default is 6 for matrices up to 64 columns (one block covers a
+// pair's outputs) and 9 above.
+namespace {
+constexpr uint32_t kMfmaCandidates[] = {9, 6, 10, 11, 4, 12};  // the cost model's choices
+}
+int mfma_catalog_size() { return (int)(sizeof(kMfmaCandidates) / sizeof(kMfmaCandidates[0])); }
+uint32_t mfma_candidate_tile(int i) { return kMfmaCandidates[i]; }
 
 namespace {
 
 template <int KIND>
 void launch_tile(const family_args& a, const float* L, const float* R, float* O, uint32_t tile, uint32_t skip_area,
                  hipStream_t s) {
+    if (tile == 0) tile = a.cols <= 64 ? 6 : 9;
     switch (tile) {
-    case 0:
-    case 4: launch<KIND, 4, 1, 128, 8, false, 2>(a, L, R, O, skip_area, s); break;
     case 1: launch<KIND, 4, 1, 128, 4, false, 2>(a, L, R, O, skip_area, s); break;
     case 2: launch<KIND, 4, 1, 256, 2, true, 1>(a, L, R, O, skip_area, s); break;
     case 3: launch<KIND, 4, 1, 128, 2, true, 1>(a, L, R, O, skip_area, s); break;
+    case 4: launch<KIND, 4, 1, 128, 8, false, 2>(a, L, R, O, skip_area, s); break;
     case 5: launch<KIND, 4, 1, 64, 8, false, 2>(a, L, R, O, skip_area, s); break;
     case 6: launch<KIND, 2, 2, 64, 4, false, 2>(a, L, R, O, skip_area, s); break;
     case 7: launch<KIND, 4, 1, 128, 8, false, 2, 2>(a, L, R, O, skip_area, s); break;
     case 8: launch<KIND, 4, 1, 128, 8, false, 1, 2>(a, L, R, O, skip_area, s); break;
+    case 9: launch_rows<KIND, 2, 128, 8, 2>(a, L, R, O, skip_area, s); break;
+    case 10: launch_rows<KIND, 2, 64, 8, 2>(a, L, R, O, skip_area, s); break;
+    case 11: launch_rows<KIND, 2, 128, 4, 2>(a, L, R, O, skip_area, s); break;
+    case 12: launch_rows<KIND, 1, 64, 8, 3>(a, L, R, O, skip_area, s); break;
     default: fail(XC_ERR_INVALID_ARGUMENT, "tile code " + std::to_string(tile) + " is not in the MFMA tile catalog");
     }
 }
@@ -867,7 +1216,7 @@ void launch_tile(const family_args& a, const float* L, const float* R, float* O,
 
 void launch_mfma_f32(const family_args& a, const float* L, const float* R, float* O, int kind, uint32_t border,
                      uint32_t tile, hipStream_t s) {
-    if (tile > 8)
+    if (tile > 12)
         fail(XC_ERR_INVALID_ARGUMENT, "tile code " + std::to_string(tile) + " is not in the MFMA tile catalog");
     switch (kind) {
     case XC_PREC_BF16X3: {

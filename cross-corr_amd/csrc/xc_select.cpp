@@ -230,7 +230,7 @@ xc_options candidate(uint32_t i) {
         o.precision = XC_PREC_FP32;
     } else if (i <= nt) {
         o.precision = XC_PREC_BF16X3;
-        o.tile = i;
+        o.tile = mfma_candidate_tile((int)i - 1);
     } else if (i >= nt + 1 && i <= nt + 3) {
         o.precision = XC_PREC_FFT64;
         o.tile = i - nt - 1;  // 0: rocFFT, 1: own FFT kernels, 2: own FFT with power-of-two rows

@@ -137,11 +137,12 @@ def test_linearity_in_the_right_matrix():
     assert (np.abs(got[2] - lin) / den[2]).max() <= 4e-6
 
 
-@pytest.mark.parametrize("tile", range(0, 9))
+@pytest.mark.parametrize("tile", range(0, 13))
 @pytest.mark.parametrize("family,rows,cols,n,m,dist", [CASES[2], CASES[7], CASES[10], CASES[13]])
 def test_every_mfma_tile(tile, family, rows, cols, n, m, dist):
-    """Tile codes 0-6 are the catalog the cost model chooses from; 7 and 8 stage two left rows
-    per block barrier (a larger ring and left double buffer) and are selectable by code."""
+    """Tile codes 1-8: the dx-split block tiles (7 and 8 stage two left rows per barrier);
+    9-12: the row-split tiles, whose waves share one output tile and take different left rows
+    (12 one sub-tile wide at three blocks per CU); 0: the default for the size."""
     left = seeded((n, rows, cols), dist, 1500 + rows, np.float32)
     right = seeded((m, rows, cols), dist, 1600 + cols, np.float32)
     err, _, _ = errors(family, left, right, run(family, left, right, precision="bf16x3", tile=tile))
@@ -151,7 +152,7 @@ def test_every_mfma_tile(tile, family, rows, cols, n, m, dist):
 def test_mfma_tile_code_out_of_catalog_is_refused():
     left = seeded((1, 16, 16), "u01", 1, np.float32)
     with pytest.raises(Exception, match="catalog"):
-        run("one_to_one", left, left, precision="bf16x3", tile=9)
+        run("one_to_one", left, left, precision="bf16x3", tile=99)
 
 
 @pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])

@@ -151,14 +151,15 @@ def test_trained_model_gain_on_held_out_workloads(x):
     fixed_t = th[:, fixed]
     gm = lambda v: float(np.exp(np.log(v).mean()))
     fp32 = cands.index("fp32/0")
-    bf = cands.index("bf16x3/4")  # the default tile
+    # the bf16x3 default tile: 6 up to 64 columns, 9 above
+    bf_of = np.array([cands.index("bf16x3/6" if int(wl[2]) <= 64 else "bf16x3/9") for wl in np.array(w)[hold]])
     both = ~np.isnan(th[:, fp32])  # rows where the definition-based methods were timed
     report = {"held_out": int(hold.sum()), "model_total_ms": model_t.sum(), "exhaustive_total_ms": best_t.sum(),
               "autotune_cost_ms": np.nansum(th, 1).sum(), "fixed_method": cands[fixed],
               "fixed_total_ms": fixed_t.sum(), "model_regret_gm": gm(model_t / best_t),
               "fixed_regret_gm": gm(fixed_t / best_t),
               "fp32_direct_over_model": th[both, fp32].sum() / model_t[both].sum(),
-              "bf16x3_default_over_model": th[both, bf].sum() / model_t[both].sum()}
+              "bf16x3_default_over_model": th[np.arange(len(th)), bf_of][both].sum() / model_t[both].sum()}
     print(report)
     assert report["model_total_ms"] <= 1.02 * report["exhaustive_total_ms"]
     assert report["model_total_ms"] <= 1.02 * report["fixed_total_ms"]
