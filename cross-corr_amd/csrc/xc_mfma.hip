@@ -286,17 +286,20 @@ __device__ __forceinline__ void mfma_tile(const float* __restrict__ Lp, const fl
     // 4j .. 4j+3 of copy t.  Right row: quad j holds columns c0 + 4j .. +3 of its ring slot.
     static_assert(G::LENA % 4 == 0 && KC % 4 == 0, "quad staging");
     constexpr int LQ = G::LENA / 4;       // quads per left copy
-    constexpr int LITEMS = 4 * LQ;        // left items per row
-    constexpr int LPT = (LITEMS + 255) / 256;
     constexpr int RQ = KC / 4;           // quads per right row
-    auto load_left = [&](int r, int cbase, int item) -> float4 {
-        const int t = item / LQ, j = item - (item / LQ) * LQ;
-        const int c = cbase + 4 * j + t;
-        float v[4];
+    static_assert(JR * LQ <= 256, "one thread per staged (left row, quad)");
+    // Left rows: thread (row i, quad j) loads the 8 values at staged positions 4j .. 4j+7,
+    // splits them once and writes quad j of all four shifted copies (as in mfma_rows_tile).
+    struct lvals {
+        float v[8];
+    };
+    auto load_left8 = [&](int r, int cbase, int j) -> lvals {
+        lvals x;
+        const int c = cbase + 4 * j;
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            v[k] = ((unsigned)r < (unsigned)H && (unsigned)(c + k) < (unsigned)W) ? Lp[(size_t)r * W + c + k] : 0.f;
-        return make_float4(v[0], v[1], v[2], v[3]);
+        for (int k = 0; k < 8; ++k)
+            x.v[k] = ((unsigned)r < (unsigned)H && (unsigned)(c + k) < (unsigned)W) ? Lp[(size_t)r * W + c + k] : 0.f;
+        return x;
     };
     auto load_right = [&](int row, int c0, int j) -> float4 {
         const int c = c0 + 4 * j;
@@ -315,13 +318,23 @@ __device__ __forceinline__ void mfma_tile(const float* __restrict__ Lp, const fl
         hi = make_uint2(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16));
         lo = make_uint2(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16));
     };
-    auto put_left = [&](int buf, int item, float4 v) {
-        const int t = item / LQ, j = item - (item / LQ) * LQ;
-        uint2 hi, lo;
-        split4(v, hi, lo);
-        char* base = lbuf + buf * G::LBUF + t * G::CS + j * 8;
-        *reinterpret_cast<uint2*>(base) = hi;
-        if (P == 2) *reinterpret_cast<uint2*>(base + 4 * G::CS) = lo;
+    auto put_left8 = [&](int buf, int j, const lvals& x) {
+        uint2 h0, l0, h1, l1;
+        split4(make_float4(x.v[0], x.v[1], x.v[2], x.v[3]), h0, l0);
+        split4(make_float4(x.v[4], x.v[5], x.v[6], x.v[7]), h1, l1);
+        auto shifted = [](uint2 a, uint2 b, int t) -> uint2 {  // elements t .. t+3 of (a, b)
+            const uint32_t w0 = a.x, w1 = a.y, w2 = b.x, w3 = b.y;
+            if (t == 0) return make_uint2(w0, w1);
+            if (t == 1) return make_uint2((w0 >> 16) | (w1 << 16), (w1 >> 16) | (w2 << 16));
+            if (t == 2) return make_uint2(w1, w2);
+            return make_uint2((w1 >> 16) | (w2 << 16), (w2 >> 16) | (w3 << 16));
+        };
+        char* base = lbuf + buf * G::LBUF + j * 8;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            *reinterpret_cast<uint2*>(base + t * G::CS) = shifted(h0, h1, t);
+            if (P == 2) *reinterpret_cast<uint2*>(base + (4 + t) * G::CS) = shifted(l0, l1, t);
+        }
     };
     auto put_right = [&](int row, int j, float4 v) {
         uint2 hi, lo;
@@ -340,9 +353,10 @@ __device__ __forceinline__ void mfma_tile(const float* __restrict__ Lp, const fl
             const int row = r_lo + JR - 1 - dyb - ro;
             put_right(row, j, load_right(row, c0, j));
         }
-        for (int i = 0; i < JR; ++i)
-            for (int item = threadIdx.x; item < LITEMS; item += 256)
-                put_left(i, item, load_left(r_lo + i, cbase, item));
+        if (threadIdx.x < JR * LQ) {
+            const int i = threadIdx.x / LQ, j = threadIdx.x - (threadIdx.x / LQ) * LQ;
+            put_left8(i, j, load_left8(r_lo + i, cbase, j));
+        }
         __syncthreads();
 
         // Wave's K chunks of this pass, and those of each dx sub-tile (offsets from c0,
@@ -365,16 +379,11 @@ __device__ __forceinline__ void mfma_tile(const float* __restrict__ Lp, const fl
             const int cur = ((r0 - r_lo) / JR) & 1;  // buffer set of this step's left rows
             const bool more = r0 + JR < r_hi;
             // Prefetch the next step's rows into registers.
-            float4 pl[JR][LPT];
+            const int li_row = threadIdx.x / LQ, lj = threadIdx.x - (threadIdx.x / LQ) * LQ;
+            const bool lstage = threadIdx.x < JR * LQ;
+            lvals pl{};
+            if (more && lstage) pl = load_left8(r0 + JR + li_row, cbase, lj);
             float4 pr = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-            for (int i = 0; i < JR; ++i)
-#pragma unroll
-                for (int k = 0; k < LPT; ++k) {
-                    const int item = threadIdx.x + k * 256;
-                    pl[i][k] = (more && item < LITEMS && r0 + JR + i < r_hi) ? load_left(r0 + JR + i, cbase, item)
-                                                                            : make_float4(0.f, 0.f, 0.f, 0.f);
-                }
             if (more && threadIdx.x < JR * RQ)
                 pr = load_right(r0 + JR + (int)threadIdx.x / RQ - dyb, c0, (int)threadIdx.x % RQ);
 
@@ -451,13 +460,7 @@ __device__ __forceinline__ void mfma_tile(const float* __restrict__ Lp, const fl
             }
 
             if (more) {
-#pragma unroll
-                for (int i = 0; i < JR; ++i)
-#pragma unroll
-                    for (int k = 0; k < LPT; ++k) {
-                        const int item = threadIdx.x + k * 256;
-                        if (item < LITEMS) put_left((cur ^ 1) * JR + i, item, pl[i][k]);
-                    }
+                if (lstage) put_left8((cur ^ 1) * JR + li_row, lj, pl);
                 if (threadIdx.x < JR * RQ)
                     put_right(r0 + JR + (int)threadIdx.x / RQ - dyb, (int)threadIdx.x % RQ, pr);
             }
