@@ -1188,10 +1188,11 @@ uint32_t batch_blocks_per_cu() { return kBOCC; }
 // default for the size.  Codes 1-8 are the dx-split block tiles (mfma_tile: WX x WY waves own
 // different outputs; KC columns per pass, FLUSH, Kahan, blocks per CU, left rows per step),
 // 9-12 the row-split tiles (mfma_rows_tile: the waves share the outputs and split the rows).
-// Measured on one MI355X (bf16x3 without the frame): 64 x 256^2 one_to_many, tile 9 2.15 ms,
-// 11 2.17, 10 2.68, 12 2.76, 4 2.79, 6 3.96; 16 x 64 of 64^2 n_to_m, tile 6 0.27 ms, 10 0.37,
-// 9 0.39, 4 0.39, 12 0.43.  The default is 6 for matrices up to 64 columns (one block covers a
-// pair's outputs) and 9 above.
+// Measured on one MI355X (bf16x3 without the frame): 64 x 256^2 one_to_many, tile 9 1.97 ms
+// (FLUSH 16 the same within run-to-run noise, at 1.5x the error), 11 2.17, 10 2.68, 12 2.76,
+// 4 2.75, 6 4.0; 16 x 64 of 64^2
+// n_to_m, tile 6 0.27 ms, 9 0.33, 10 0.37, 4 0.38, 12 0.43.  The default is 6 for matrices up
+// to 64 columns (one block covers a pair's outputs) and 9 above.
 namespace {
 constexpr uint32_t kMfmaCandidates[] = {9, 6, 10, 11, 4, 12};  // the cost model's choices
 }
