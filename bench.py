@@ -12,7 +12,8 @@ error (|x - x64| / corr(|L|,|R|)) is measured on results 0 and 63 against the FP
 
 `configs` holds the other workloads, each with its own timing, error and roofline:
   c1  one_to_one 128x128 float32 U[0,1)
-  c3  n_to_m 16 left x 64 right 64x64 float32 N(0,1), and the same data in float64
+  c3  n_to_m 16 left x 64 right 64x64 float32 N(0,1) (FP32 direct, bf16x3, fp16x3, single
+      bf16 and fp16 MFMA, the own FFT, AUTO), and the same data in float64
   c4  uneven n_to_mn batch: 128 groups of 1 left + 4 right square matrices of size
       round(16 * 32^U) (seed 2024), float32 U[-1,1), 512 pairs: one AUTO batch call (the
       schedule the batch autotuner measured fastest, replayed from a HIP graph) against one
@@ -50,6 +51,7 @@ HBM_GBPS = 8000.0
 # FP64 VALU, dense BF16/FP16 MFMA.
 TFLOPS = {"direct_kernel": 78.6, "direct_kernel_f64": 78.6, "mfma_kernel": 2500.0, "batch_kernel": 2500.0}
 METHODS = {"fp32": dict(precision="fp32"), "bf16x3": dict(precision="bf16x3"), "fp16x3": dict(precision="fp16x3"),
+           "bf16": dict(precision="bf16"), "fp16": dict(precision="fp16"),
            "fft64": dict(precision="fft64", tile=0), "fft64k": dict(precision="fft64", tile=1),
            "auto": dict(precision="auto")}
 
@@ -347,7 +349,7 @@ def main():
         r3 = seeded((64, 64, 64), "normal", 4, np.float32)
         chk = [0, 63, 64 * 7 + 5, 1023]
         c3 = family_config("c3", "n_to_m 16 x 64 of 64x64 float32 N(0,1)", "n_to_m", l3, r3,
-                           ["fp32", "bf16x3", "fp16x3", "fft64k", "auto"], chk, args.steps)
+                           ["fp32", "bf16x3", "fp16x3", "bf16", "fp16", "fft64k", "auto"], chk, args.steps)
         d3 = family_config("c3-f64", "the same data in float64 (-d double)", "n_to_m", l3.astype(np.float64),
                            r3.astype(np.float64), ["fp32", "fft64k", "auto"], chk, args.steps)
         d3["methods"]["fp32"]["note"] = "float64 data: the FP64 direct kernel"
