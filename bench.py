@@ -453,6 +453,16 @@ def main():
                 "pairs_per_s": a["pairs_per_s"], "max_norm_err": a["max_norm_err"],
                 "error_sample": "results 0, 63, 127 at 6 shifts (corners, centre, interior) against the definition",
                 "methods": res}
+        # every output of the AUTO result against the own FFT on the data in float64 (its error
+        # is ~1e-16 of the largest output), reduced on the GPU in FP64; U[0,1) data:
+        # corr(|L|,|R|) is the FP64 reference itself
+        x.cross_corr("one_to_many", L, R, out=o5, precision="auto")
+        ref5 = x.cross_corr("one_to_many", L.double(), R.double(), precision="fft64", tile=1)
+        mx, mean, _ = x.norm_error(o5, ref5, ref5)
+        del ref5
+        line["all_outputs_vs_fp64_fft"] = {"max_norm_err": mx, "mean_norm_err": mean,
+                                           "outputs": M * (2 * N - 1) ** 2}
+        log(f"c5 auto, all outputs against the FP64 own FFT: max {mx:.2e}, mean {mean:.2e}")
         t = kernels_of(lambda: x.cross_corr("one_to_many", L, R, out=o5, precision="auto"), 3)
         pow2 = line["auto_selected"] == "fft64/2"
         line["roofline"] = roofline(t, lambda kk: kernel_bytes(kk, N, N, 1, M, M, pow2),

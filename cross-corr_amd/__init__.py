@@ -126,6 +126,10 @@ SIGNATURES = {
                               ctypes.c_float, ctypes.c_float, _vp]),
     "xc_mlp_train_f32": (_st, [ctypes.POINTER(MlpShape), _vp, _vp, _vp, _u32, ctypes.POINTER(TrainOptions), _vp,
                                _vp]),
+    "xc_validate_result_f32": (_st, [_vp, _vp, _u64, ctypes.POINTER(ctypes.c_double), _vp]),
+    "xc_validate_result_f64": (_st, [_vp, _vp, _u64, ctypes.POINTER(ctypes.c_double), _vp]),
+    "xc_norm_error_f32": (_st, [_vp, _vp, _vp, _u64, ctypes.POINTER(ctypes.c_double), _vp]),
+    "xc_norm_error_f64": (_st, [_vp, _vp, _vp, _u64, ctypes.POINTER(ctypes.c_double), _vp]),
     "xc_kernel_timing": (_st, [_i32]),
     "xc_kernel_times": (_st, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "xc_last_error": (ctypes.c_char_p, []),
@@ -276,6 +280,47 @@ def pair_array(pairs):
     if isinstance(pairs, ctypes.Array):
         return pairs
     return (Pair * len(pairs))(*[Pair(*p) for p in pairs])
+
+
+def _check_same(tensors, dtypes, what):
+    """Contiguous GPU tensors on one device with the same element count, of the given dtypes."""
+    import torch
+    if not all(isinstance(t, torch.Tensor) for t in tensors):
+        raise XcError(1, f"{what}: inputs must be torch tensors")
+    if not all(t.is_cuda for t in tensors) or len({t.device for t in tensors}) != 1:
+        raise XcError(1, f"{what}: inputs must be GPU tensors on one device")
+    if len({t.numel() for t in tensors}) != 1:
+        raise XcError(1, f"{what}: inputs must have the same number of elements")
+    for t, d in zip(tensors, dtypes):
+        if t.dtype not in d:
+            raise XcError(1, f"{what}: unexpected dtype {t.dtype}")
+    return [t.contiguous() for t in tensors]
+
+
+def validate_result(result, valid, stream=None):
+    """The reference's validate_result statistics (src/validate.hpp:134-188) reduced on the
+    GPU in FP64: {max, mean, stddev, result_at_max, valid_at_max}.  result and valid: GPU
+    tensors of one dtype (float32 or float64) with the same number of elements."""
+    import torch
+    result, valid = _check_same([result, valid], [(torch.float32, torch.float64)] * 2, "validate_result")
+    if result.dtype != valid.dtype:
+        raise XcError(1, "validate_result: result and valid must have one dtype")
+    out = (ctypes.c_double * 5)()
+    fn = lib().xc_validate_result_f32 if result.dtype == torch.float32 else lib().xc_validate_result_f64
+    check(fn(_ptr(result), _ptr(valid), result.numel(), out, _stream(stream)))
+    return dict(zip(("max", "mean", "stddev", "result_at_max", "valid_at_max"), list(out)))
+
+
+def norm_error(x, ref, den, stream=None):
+    """Normalised error |x - ref| / den reduced on the GPU in FP64: (max, mean, index of the
+    max).  x: float32 or float64; ref and den (corr(|L|, |R|)): float64; same element count."""
+    import torch
+    x, ref, den = _check_same([x, ref, den], [(torch.float32, torch.float64), (torch.float64,), (torch.float64,)],
+                              "norm_error")
+    out = (ctypes.c_double * 3)()
+    fn = lib().xc_norm_error_f32 if x.dtype == torch.float32 else lib().xc_norm_error_f64
+    check(fn(_ptr(x), _ptr(ref), _ptr(den), x.numel(), out, _stream(stream)))
+    return out[0], out[1], int(out[2])
 
 
 def batch(pairs, left_flat, right_flat, out_flat, precision="auto", tile=0, split=0, stream=None):

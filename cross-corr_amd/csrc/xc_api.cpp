@@ -514,4 +514,36 @@ xc_status xc_device_info(char* buf, size_t len) {
     });
 }
 
+xc_status xc_validate_result_f32(const float* result, const float* valid, uint64_t count, double* out,
+                                 xc_stream_t stream) {
+    return guarded([&] {
+        if (!out || (count && (!result || !valid))) xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        xc::validate_result_f32(result, valid, count, out, (hipStream_t)stream);
+    });
+}
+
+xc_status xc_validate_result_f64(const double* result, const double* valid, uint64_t count, double* out,
+                                 xc_stream_t stream) {
+    return guarded([&] {
+        if (!out || (count && (!result || !valid))) xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        xc::validate_result_f64(result, valid, count, out, (hipStream_t)stream);
+    });
+}
+
+xc_status xc_norm_error_f32(const float* x, const double* ref, const double* den, uint64_t count, double* out,
+                            xc_stream_t stream) {
+    return guarded([&] {
+        if (!out || (count && (!x || !ref || !den))) xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        xc::norm_error_f32(x, ref, den, count, out, (hipStream_t)stream);
+    });
+}
+
+xc_status xc_norm_error_f64(const double* x, const double* ref, const double* den, uint64_t count, double* out,
+                            xc_stream_t stream) {
+    return guarded([&] {
+        if (!out || (count && (!x || !ref || !den))) xc::fail(XC_ERR_INVALID_ARGUMENT, "null pointer");
+        xc::norm_error_f64(x, ref, den, count, out, (hipStream_t)stream);
+    });
+}
+
 }  // extern "C"

@@ -232,6 +232,26 @@ xc_status xc_fft_cross_corr_f64(xc_family family, const double* left, const doub
                                 uint32_t rows, uint32_t cols, uint32_t n, uint32_t m, int normalize,
                                 xc_stream_t stream);
 
+/* ---- on-device error checks (FP64 reductions) -----------------------------------------
+ * xc_validate_result_*: the statistics of the reference's validate_result
+ * (src/validate.hpp:134-188) over count elements of device arrays: per element d = 0 when
+ * |result| and |valid| are both at most 1e-5, else |result - valid| / max(|result|, |valid|);
+ * out[0..4] = max d (first index on ties), mean, population standard deviation, and the
+ * result and valid values at the max.  xc_norm_error_*: the normalised error this library
+ * reports for a result x against an FP64 reference ref, e = |x - ref| / den with den =
+ * corr(|L|, |R|) (e = 0 where den = 0 and x = ref, infinite where den = 0 otherwise);
+ * out[0..2] = max e, mean e, index of the max (no counterpart in the reference, whose
+ * checks run on the host).  Both read their results back to the host: they synchronise
+ * `stream`. */
+xc_status xc_validate_result_f32(const float* result, const float* valid, uint64_t count, double* out,
+                                 xc_stream_t stream);
+xc_status xc_validate_result_f64(const double* result, const double* valid, uint64_t count, double* out,
+                                 xc_stream_t stream);
+xc_status xc_norm_error_f32(const float* x, const double* ref, const double* den, uint64_t count, double* out,
+                            xc_stream_t stream);
+xc_status xc_norm_error_f64(const double* x, const double* ref, const double* den, uint64_t count, double* out,
+                            xc_stream_t stream);
+
 /* ---- reference run_* launchers with their tuning arguments --------------------------
  * Same argument meaning as src/kernels.cuh:65-73 and :223-236; tuning arguments are
  * validated with the limits of the reference's release build (warps_per_thread_block <= 32,

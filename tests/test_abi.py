@@ -114,3 +114,17 @@ def test_python_binding_checks_a_given_out():
         x._check_out(torch.zeros(shape, dtype=torch.float64), shape, like)
     with pytest.raises(x.XcError, match="dtype"):
         x._check_out(ok.numpy(), shape, like)
+
+
+def test_device_checks_refuse_bad_inputs():
+    """The on-device error checks validate their tensors before any GPU call."""
+    torch = pytest.importorskip("torch")
+    x = xcorr()
+    a = torch.zeros(4, dtype=torch.float64)
+    with pytest.raises(x.XcError, match="GPU"):
+        x.validate_result(a, a)
+    with pytest.raises(x.XcError, match="GPU"):
+        x.norm_error(a, a, a)
+    with pytest.raises(x.XcError, match="torch"):
+        x.validate_result(a.numpy(), a)
+
