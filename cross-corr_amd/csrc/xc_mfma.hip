@@ -825,13 +825,6 @@ __global__ __launch_bounds__(256, OCC) void mfma_rows_kernel(const float* __rest
                                          O + (size_t)k * (2 * W - 1) * (2 * H - 1), H, W, tx, ty, skip_area, smem);
 }
 
-// Frame outputs of a (rows, cols, threshold): the overlap sizes (h, w) with h*w < threshold,
-// enumerated h-major; start[h-1] is the index of (h, 1) and start[nh] the total count.
-struct frame_table {
-    uint32_t nh;
-    uint32_t start[kMaxBorder + 1];
-};
-
 // Value of one output (dy, dx) with overlap h x w, computed by one thread: products of FP32
 // values are exact in FP64 and their FP64 sum carries a relative error below h*w*2^-53, so the
 // result is the exact sum rounded once to FP32.
@@ -859,137 +852,213 @@ __device__ __forceinline__ void frame_hw(const uint32_t* start, uint32_t nh, uin
 }
 
 // Exact recomputation of the outputs whose overlap area h*w (h = H-|dy|, w = W-|dx|) is
-// below the threshold, in tasks: a task is (quadrant, height h, group of up to kFrameGroup
-// consecutive widths), and the lanes of a wave run one task for LP consecutive pairs (64/LP
-// tasks per wave), so that every lane of a task has the same trip count.  The matrices are
-// read from padded interleaved copies (element (r, c) of matrix i at ((r * Wp + c) * count + i),
-// Wp = W + kFramePad, zero for c >= W), so that the lanes of a task, which take consecutive
-// pairs, read consecutive addresses, and reads past a row end need no test.
-//
-// Every quadrant reads as out(s) = sum_{i<h} sum_{c<w} A[ra+i][s+c] * B[rb+i][c] with s = W-w:
-// (A, B) = (L, R) where dx >= 0 and (R, L) where dx < 0; rows ra, rb are H-h and 0 in the
-// order (L, R) where dy >= 0, 0 and H-h where dy < 0.  The group's outputs (s = s0 + k,
-// k < kFrameGroup) take the columns c in steps of kFrameGroup: step (i, g) reads A[ra+i][s0 + 8g
-// + j] for j < 2 kFrameGroup and B[rb+i][8g + j] for j < kFrameGroup, and adds A[..][s0 + 8g +
-// j + k] * B[..][8g + j] to output k; A is zero past the row end, which ends each output's sum
-// at its own width.  The steps run as one flat sequence over (i, g) with the next step's
-// values loaded while the current one is summed.  Products of FP32 values are exact in FP64,
-// so each output is its exact sum rounded once.
+// below the threshold T.  Every output reads as
+//   out = sum_{i<h} sum_{c<w} L[rl+i][cl+c] * R[rr+i][cr+c],
+// (rl, rr) = (H-h, 0) where dy >= 0, (0, H-h) where dy < 0; (cl, cr) likewise from dx and w.
+// The frame is cut into tasks of up to kFrameGroup outputs that share one loop: a fixed h
+// and consecutive widths w = whi-k.  A (the matrix whose column offset moves with w: L
+// where dx >= 0, else R) is read from column s0 = W-whi on; output k takes
+// A[ra+i][s0+k+c] * B[rb+i][c] over c < ceil(whi/G)*G, A being zero past its last column,
+// so each output's sum ends at its own width.  Heights up to Kh (the largest h with h*h < T)
+// run this way on row-major copies of the matrices; taller outputs (w < T/Kh) run the same
+// code on column-major copies, where they are short and wide.  No task runs more than about
+// T/G + sqrt(T) steps.
+// A step (row i, group v of G columns) loads 4 quads of A (from column s0 + G*v, which the
+// table keeps a multiple of 4 by aligning each group's top width) and 2 quads of B, and does
+// G*G FP64 FMAs, with the next step's loads in flight.
+// The copies hold quads of 4 consecutive elements of a row, interleaved over the matrices
+// (quad q of row r of matrix i at ((r * Q + q) * count + i) * 4), zero past the row end, so
+// the lanes, which take consecutive pairs, read 16 consecutive bytes each from one contiguous
+// range, and nothing needs a bounds test.  The lanes of a wave run one task for LP
+// consecutive pairs (LP = 64: the task is uniform over the wave and its addressing scalar;
+// fewer pairs: 64/LP tasks per wave).
+// Products of FP32 values are exact in FP64, so each output is its exact sum rounded once.
 constexpr int kFrameGroup = 8;
-constexpr int kFramePad = 2 * kFrameGroup;
+constexpr int kFrameEntries = 48;  // at most 2 * 22 entries for T <= 512
 
-// Padded interleaved copy: X holds count matrices of rows x cols; Xt[(r * (cols + kFramePad) +
-// c) * count + i] = X[i][r][c], zero for c >= cols.  32 x 32 tiles (elements x matrices)
-// through LDS, so that reads and writes are both coalesced.
-__global__ __launch_bounds__(256) void interleave_kernel(const float* __restrict__ X, float* __restrict__ Xt,
-                                                         uint32_t count, uint32_t cols, uint64_t padded_area) {
-    __shared__ float tile[32][33];
-    const uint64_t e0 = (uint64_t)blockIdx.x * 32;  // padded element index r * Wp + c
-    const uint32_t i0 = blockIdx.y * 32;
-    const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    const uint32_t wp = cols + kFramePad;
-    for (uint32_t k = ty; k < 32; k += 8) {  // matrix i0 + k, padded element e0 + tx
-        const uint32_t i = i0 + k;
-        const uint64_t e = e0 + tx;
-        const uint64_t r = e / wp;
-        const uint32_t c = (uint32_t)(e - r * wp);
-        tile[k][tx] = (i < count && e < padded_area && c < cols) ? X[((uint64_t)i * (padded_area / wp) + r) * cols + c]
-                                                                 : 0.f;
+// Padded row length of the copies: the A window of the last group ends at most 14 columns
+// past the last column; a multiple of 4.
+__host__ __device__ constexpr uint32_t frame_padded(uint32_t len) { return (len + 15 + 3) / 4 * 4; }
+
+// Task table: entry e owns tasks [start[e], start[e+1]) of each quadrant; fixed[e] is the
+// fixed height (bit 31 set: on the column-major copies, the fixed width), the entry's widths
+// (heights) are vlo[e] .. vmax[e], and its first task's group ends at top[e] (the smallest
+// value >= vmax congruent to the row length mod 4); each next task's group ends G lower.
+struct frame_plan {
+    uint32_t ne;
+    uint32_t start[kFrameEntries + 1];
+    uint32_t fixed[kFrameEntries];
+    uint32_t vlo[kFrameEntries];
+    uint32_t vmax[kFrameEntries];
+    uint32_t top[kFrameEntries];
+};
+
+// The two quad copies of count matrices of rows x cols: row-major Xn (rows x Wp, Wp =
+// frame_padded(cols)) and column-major Xc (cols x Hp), zero past each row's end.  A block
+// stages 16 matrices x 16 x 16 elements in LDS and writes 16-byte quads, the 16 matrices of
+// a quad position contiguous.
+__global__ __launch_bounds__(256) void frame_copy_kernel(const float* __restrict__ X, float* __restrict__ Xn,
+                                                         float* __restrict__ Xc, uint32_t count, uint32_t rows,
+                                                         uint32_t cols, uint32_t tiles_c) {
+    __shared__ float tile[16][16][17];
+    const uint32_t Wp = frame_padded(cols), Hp = frame_padded(rows);
+    const uint32_t r0 = (blockIdx.x / tiles_c) * 16, c0 = (blockIdx.x % tiles_c) * 16;
+    const uint32_t i0 = blockIdx.y * 16;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t e = threadIdx.x + 256 * k;
+        const uint32_t mt = e >> 8, rr = (e >> 4) & 15, cc = e & 15;
+        const uint32_t i = i0 + mt, r = r0 + rr, c = c0 + cc;
+        tile[mt][rr][cc] = (i < count && r < rows && c < cols) ? X[((size_t)i * rows + r) * cols + c] : 0.f;
     }
     __syncthreads();
-    for (uint32_t k = ty; k < 32; k += 8) {  // padded element e0 + k of matrices i0 + tx
-        const uint32_t i = i0 + tx;
-        const uint64_t e = e0 + k;
-        if (i < count && e < padded_area) Xt[e * count + i] = tile[tx][k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t e = threadIdx.x + 256 * k;
+        const uint32_t mt = e & 15, qd = (e >> 4) & 3, rr = e >> 6;  // matrix, quad, row
+        const uint32_t i = i0 + mt;
+        if (i >= count) continue;
+        const uint32_t r = r0 + rr, c = c0 + 4 * qd;  // row-major quad (r, c .. c+3)
+        if (r < rows && c < Wp) {
+            const float4 v = make_float4(tile[mt][rr][4 * qd], tile[mt][rr][4 * qd + 1], tile[mt][rr][4 * qd + 2],
+                                         tile[mt][rr][4 * qd + 3]);
+            *reinterpret_cast<float4*>(Xn + (((size_t)r * (Wp / 4) + c / 4) * count + i) * 4) = v;
+        }
+        const uint32_t cr = c0 + rr, rq = r0 + 4 * qd;  // column-major quad (column cr, rows rq .. rq+3)
+        if (cr < cols && rq < Hp) {
+            const float4 v = make_float4(tile[mt][4 * qd][rr], tile[mt][4 * qd + 1][rr], tile[mt][4 * qd + 2][rr],
+                                         tile[mt][4 * qd + 3][rr]);
+            *reinterpret_cast<float4*>(Xc + (((size_t)cr * (Hp / 4) + rq / 4) * count + i) * 4) = v;
+        }
     }
 }
 
-__global__ __launch_bounds__(256) void frame_kernel(const float* __restrict__ Lt, const float* __restrict__ Rt,
+template <bool WAVE>
+__device__ __forceinline__ uint32_t frame_uniform(uint32_t v) {
+    if constexpr (WAVE) return __builtin_amdgcn_readfirstlane(v);
+    else return v;
+}
+
+// The quad at base + lane_bytes + uniform_bytes.  With a wave-uniform base (WAVE) this is a
+// raw buffer load: the base in a scalar resource, the lane's offset in a VGPR and the
+// uniform one in an SGPR, so no per-lane address arithmetic.  The host guarantees that the
+// offsets fit 31 bits.
+template <bool WAVE>
+__device__ __forceinline__ void frame_load4(const float* __restrict__ base, uint32_t lane_bytes,
+                                            uint32_t uniform_bytes, float* v) {
+    if constexpr (WAVE) {
+        const __amdgpu_buffer_rsrc_t rsrc =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0, 0xffffffff, 0x00020000);
+        typedef int i32x4 __attribute__((ext_vector_type(4)));
+        const i32x4 raw = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)lane_bytes, (int)uniform_bytes, 0);
+        const float4 q = __builtin_bit_cast(float4, raw);
+        v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
+    } else {
+        const float4 q =
+            *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + lane_bytes + uniform_bytes);
+        v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
+    }
+}
+
+template <bool WAVE>
+__global__ __launch_bounds__(256) void frame_kernel(const float* __restrict__ Ln, const float* __restrict__ Rn,
+                                                    const float* __restrict__ Lc, const float* __restrict__ Rc,
                                                     float* __restrict__ O, int family, uint32_t rows,
                                                     uint32_t cols, uint32_t n, uint32_t m, uint32_t nl_mats,
-                                                    uint32_t nr_mats, uint32_t pairs, uint32_t thresh,
-                                                    uint32_t lp_shift, frame_table tab) {
+                                                    uint32_t nr_mats, uint32_t pairs, uint32_t lp_shift,
+                                                    frame_plan tab) {
     constexpr int G = kFrameGroup;
-    __shared__ uint32_t start[kMaxBorder + 1];
-    for (uint32_t i = threadIdx.x; i <= tab.nh; i += blockDim.x) start[i] = tab.start[i];
-    __syncthreads();
-    const int H = (int)rows, W = (int)cols, Wp = W + kFramePad;
-    const uint32_t per_quad = start[tab.nh];
-    const uint32_t slot = (blockIdx.x * blockDim.x + threadIdx.x) >> lp_shift;
+    const uint32_t per_quad = tab.start[tab.ne];
+    const uint32_t slot = frame_uniform<WAVE>((blockIdx.x * blockDim.x + threadIdx.x) >> lp_shift);
     if (slot >= 4 * per_quad) return;
     const uint32_t q = slot / per_quad, t = slot - q * per_quad;
-    int lo = 0, hi = (int)tab.nh - 1;  // height index: largest with start[.] <= t
+    int lo = 0, hi = (int)tab.ne - 1;  // entry: the largest with start[.] <= t
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= t) lo = mid;
+        if (tab.start[mid] <= t) lo = mid;
         else hi = mid - 1;
     }
-    const int h = lo + 1;
-    const int cnt = min(W, (int)((thresh + h - 1) / h) - 1);
-    const int wlo = (int)(t - start[lo]) * G + 1;
-    int whi = min(cnt, wlo + G - 1);
-    const bool dyneg = q & 1, dxneg = q & 2;
-    if (dxneg) whi = min(whi, W - 1);     // dx = 0 belongs to the dx >= 0 quadrants
-    if ((dyneg && h == H) || wlo > whi) return;  // dy = 0 belongs to the dy >= 0 quadrants
-    const int rl = dyneg ? 0 : H - h, rr = dyneg ? H - h : 0;
-    const int ra = dxneg ? rr : rl, rb = dxneg ? rl : rr;
-    const int s0 = W - whi;  // output k: shift s0 + k, width whi - k
-    const uint32_t sa = dxneg ? nr_mats : nl_mats, sb = dxneg ? nl_mats : nr_mats;  // interleave strides
-    const int ng = (whi + G - 1) / G;  // column steps per row
+    const uint32_t e = frame_uniform<WAVE>((uint32_t)lo);
+    const bool tr = tab.fixed[e] >> 31;  // on the column-major copies
+    const int h = (int)(tab.fixed[e] & 0x7fffffffu);
+    const int whi = (int)(tab.top[e] - (t - tab.start[e]) * G);
+    const int vlo = (int)tab.vlo[e], vmax = (int)tab.vmax[e];
+    // the task's own frame: rows Hd x columns Wd, and the signs of its row and column shifts
+    const int Hd = tr ? (int)cols : (int)rows, Wd = tr ? (int)rows : (int)cols;
+    const bool rneg = tr ? (q & 2) : (q & 1), cneg = tr ? (q & 1) : (q & 2);
+    if (rneg && h == Hd) return;  // shift 0 belongs to the non-negative quadrants
+    const bool a_left = !cneg;
+    const int rl = rneg ? 0 : Hd - h, rr = rneg ? Hd - h : 0;
+    const int ra = a_left ? rl : rr, rb = a_left ? rr : rl;
+    const int s0 = Wd - whi;  // a multiple of 4
+    const int Qd = (int)frame_padded((uint32_t)Wd) / 4;  // quads per row of the copy
+    const float* __restrict__ Ab = a_left ? (tr ? Lc : Ln) : (tr ? Rc : Rn);
+    const float* __restrict__ Bb = a_left ? (tr ? Rc : Rn) : (tr ? Lc : Ln);
+    const int64_t sa = a_left ? nl_mats : nr_mats, sb = a_left ? nr_mats : nl_mats;  // matrices per quad position
+    const int ng = (whi + G - 1) / G;
     const int nsteps = h * ng;
-    const int SY = 2 * H - 1, SX = 2 * W - 1;
-    const int dy = dyneg ? -(H - h) : H - h;
+    const int SY = 2 * (int)rows - 1, SX = 2 * (int)cols - 1;
+    const uint32_t lane_mask = (1u << lp_shift) - 1;
     const uint32_t nchunks = ((pairs - 1) >> lp_shift) + 1;
     for (uint32_t chunk = blockIdx.y; chunk < nchunks; chunk += gridDim.y) {
-        const uint32_t pk = (chunk << lp_shift) + (threadIdx.x & ((1u << lp_shift) - 1));
-        if (pk >= pairs) break;
+        const uint32_t pk = (chunk << lp_shift) + (threadIdx.x & lane_mask);
+        const bool valid = pk < pairs;  // lanes past the last pair run the last one and store nothing
         uint32_t li, ri;
-        pair_of(family, n, m, pk, li, ri);
-        const float* __restrict__ A = (dxneg ? Rt : Lt) + (dxneg ? ri : li);
-        const float* __restrict__ B = (dxneg ? Lt : Rt) + (dxneg ? li : ri);
-        // element offsets (before the interleave stride) of step (i, g)
-        auto fetch = [&](int i, int g, float (&va)[2 * G], float (&vb)[G]) {
-            const float* __restrict__ ap = A + ((size_t)(ra + i) * Wp + s0 + G * g) * sa;
-            const float* __restrict__ bp = B + ((size_t)(rb + i) * Wp + G * g) * sb;
+        pair_of(family, n, m, valid ? pk : pairs - 1, li, ri);
+        const uint32_t la = 16 * (a_left ? li : ri), lb = 16 * (a_left ? ri : li);  // lane byte offsets
+        // quads of step (i, v): A row ra+i from column s0 + G*v, B row rb+i from column G*v
+        auto fetch = [&](int row, int grp, float (&va)[2 * G], float (&vb)[G]) {
+            const float* __restrict__ ap = Ab + ((int64_t)(ra + row) * Qd + (s0 + G * grp) / 4) * sa * 4;
+            const float* __restrict__ bp = Bb + ((int64_t)(rb + row) * Qd + G * grp / 4) * sb * 4;
 #pragma unroll
-            for (int j = 0; j < 2 * G; ++j) va[j] = ap[(size_t)j * sa];
+            for (int j = 0; j < G / 2; ++j) frame_load4<WAVE>(ap, la, (uint32_t)(16 * j * sa), va + 4 * j);
 #pragma unroll
-            for (int j = 0; j < G; ++j) vb[j] = bp[(size_t)j * sb];
+            for (int j = 0; j < G / 4; ++j) frame_load4<WAVE>(bp, lb, (uint32_t)(16 * j * sb), vb + 4 * j);
         };
         double acc[G];
 #pragma unroll
         for (int k = 0; k < G; ++k) acc[k] = 0.0;
-        float ca[2 * G], cb[G];
-        fetch(0, 0, ca, cb);
-        int i = 0, g = 0;
-        for (int st = 0; st < nsteps; ++st) {
-            if (++g == ng) {
-                g = 0;
-                ++i;
-            }
-            float na[2 * G], nb[G];
-            if (st + 1 < nsteps) fetch(i, g, na, nb);  // the next step, in flight during this one
-            double da[2 * G];
+        auto sum = [&](const float (&va)[2 * G], const float (&vb)[G]) {
+            double da[2 * G - 1];  // A value 2G-1 is never used
 #pragma unroll
-            for (int j = 0; j < 2 * G; ++j) da[j] = (double)ca[j];
+            for (int j = 0; j < 2 * G - 1; ++j) da[j] = (double)va[j];
 #pragma unroll
             for (int j = 0; j < G; ++j) {
-                const double bv = (double)cb[j];
+                const double bv = (double)vb[j];
 #pragma unroll
                 for (int k = 0; k < G; ++k) acc[k] = fma(da[j + k], bv, acc[k]);
             }
-            if (st + 1 < nsteps) {
-#pragma unroll
-                for (int j = 0; j < 2 * G; ++j) ca[j] = na[j];
-#pragma unroll
-                for (int j = 0; j < G; ++j) cb[j] = nb[j];
+        };
+        // two register sets in turn: each step's successor is loaded while it is summed
+        float xa[2 * G], xb[G], ya[2 * G], yb[G];
+        int i = 0, v = 0;
+        auto next = [&]() {
+            if (++v == ng) {
+                v = 0;
+                ++i;
             }
+        };
+        fetch(0, 0, xa, xb);
+        for (int st = 0; st < nsteps; st += 2) {
+            next();
+            if (st + 1 < nsteps) fetch(i, v, ya, yb);
+            sum(xa, xb);
+            if (st + 1 == nsteps) break;
+            next();
+            if (st + 2 < nsteps) fetch(i, v, xa, xb);
+            sum(ya, yb);
         }
-        float* __restrict__ op = O + (size_t)pk * SX * SY + (size_t)(dy + H - 1) * SX;
+        if (valid) {
+            float* __restrict__ op = O + (size_t)pk * SX * SY;
+            const int dr = rneg ? -(Hd - h) : Hd - h;
 #pragma unroll
-        for (int k = 0; k < G; ++k) {
-            const int w = whi - k;
-            if (w >= wlo) op[(dxneg ? -(W - w) : W - w) + W - 1] = (float)acc[k];
+            for (int k = 0; k < G; ++k) {
+                const int x = whi - k;  // the output's width in the task's frame
+                if (x < vlo || x > vmax || (cneg && x == Wd)) continue;
+                const int dc = cneg ? -(Wd - x) : Wd - x;
+                const int dy = tr ? dc : dr, dx = tr ? dr : dc;
+                op[(size_t)(dy + (int)rows - 1) * SX + (dx + (int)cols - 1)] = (float)acc[k];
+            }
         }
     }
 }
@@ -1041,44 +1110,72 @@ void launch_rows(const family_args& a, const float* L, const float* R, float* O,
 void launch_frame_f32(const family_args& a, const float* L, const float* R, float* O, uint32_t thresh, hipStream_t s) {
     if (a.pairs == 0 || thresh <= 1) return;
     if (thresh > (uint32_t)kMaxBorder) fail(XC_ERR_INVALID_ARGUMENT, "border threshold above " + std::to_string(kMaxBorder));
-    // Tasks per quadrant: for each height h with frame outputs, ceil(widths / kFrameGroup).
-    frame_table tab{};
-    uint32_t total = 0, nh = 0;
-    for (uint32_t h = 1; h <= a.rows && h < thresh; ++h) {
-        const uint32_t cnt = std::min<uint32_t>(a.cols, (thresh + h - 1) / h - 1);
-        if (cnt == 0) break;
-        tab.start[nh++] = total;
-        total += (cnt + kFrameGroup - 1) / kFrameGroup;
+    // Task table: heights 1 .. Kh (the largest h with h * h < T, at most rows) on the
+    // row-major copies, then the widths that have frame outputs of height above Kh on the
+    // column-major copies.  Each entry's groups of widths (heights) end at top, top - G, ...
+    frame_plan tab{};
+    uint32_t total = 0, ne = 0;
+    auto add = [&](bool column_major, uint32_t fixed, uint32_t vlo, uint32_t vmax, uint32_t row_len) {
+        if (ne == (uint32_t)kFrameEntries) fail(XC_ERR_INVALID_ARGUMENT, "border frame: task table overflow");
+        const uint32_t top = vmax + (row_len - vmax) % 4;  // <= row_len, row_len - top a multiple of 4
+        tab.start[ne] = total;
+        tab.fixed[ne] = fixed | (column_major ? 0x80000000u : 0u);
+        tab.vlo[ne] = vlo;
+        tab.vmax[ne] = vmax;
+        tab.top[ne] = top;
+        ++ne;
+        total += (top - vlo + kFrameGroup) / kFrameGroup;
+    };
+    uint32_t kh = 0;
+    while ((kh + 1) * (kh + 1) < thresh) ++kh;
+    kh = std::min(kh, a.rows);
+    for (uint32_t h = 1; h <= kh; ++h) add(false, h, 1, std::min<uint32_t>(a.cols, (thresh + h - 1) / h - 1), a.cols);
+    for (uint32_t w = 1; w <= a.cols; ++w) {
+        const uint32_t hmax = std::min<uint32_t>(a.rows, (thresh + w - 1) / w - 1);
+        if (hmax <= kh) break;
+        add(true, w, kh + 1, hmax, a.rows);
     }
-    if (nh == 0) return;
-    tab.nh = nh;
-    tab.start[nh] = total;
-    // Padded interleaved copies of the left and right matrices.
+    if (ne == 0) return;
+    tab.ne = ne;
+    tab.start[ne] = total;
+    // Row-major and column-major quad copies of the left and right matrices.
     const uint32_t nl = a.family == XC_ONE_TO_ONE || a.family == XC_ONE_TO_MANY ? 1 : a.n;
     const uint32_t nr = a.family == XC_ONE_TO_ONE ? 1 : a.m;
-    const uint64_t padded = (uint64_t)a.rows * (a.cols + kFramePad);
-    float* next = static_cast<float*>(workspace(WS_FRAME, sizeof(float) * padded * (nl + nr), s));
-    auto interleave = [&](const float* X, uint32_t count) {
-        float* Xt = next;
-        next += padded * count;
-        const uint64_t gx = (padded + 31) / 32;
-        const uint32_t gy = (count + 31) / 32;
-        if (gx > 0x7fffffffULL || gy > 65535) fail(XC_ERR_INVALID_ARGUMENT, "border frame: matrices too large to interleave");
-        hipLaunchKernelGGL(interleave_kernel, dim3((uint32_t)gx, gy), dim3(256), 0, s, X, Xt, count, a.cols, padded);
+    const uint32_t Wp = frame_padded(a.cols), Hp = frame_padded(a.rows);
+    const uint64_t per_mat = (uint64_t)a.rows * Wp + (uint64_t)a.cols * Hp;
+    float* next = static_cast<float*>(workspace(WS_FRAME, sizeof(float) * per_mat * (nl + nr), s));
+    auto copy = [&](const float* X, uint32_t count, const float*& Xn, const float*& Xc) {
+        float* xn = next;
+        float* xc = next + (uint64_t)a.rows * Wp * count;
+        next += per_mat * count;
+        const uint64_t tiles_c = (Wp + 15) / 16, tiles = (uint64_t)((Hp + 15) / 16) * tiles_c;
+        const uint64_t gy = (count + 15) / 16;
+        if (tiles > 0x7fffffffULL || gy > 65535) fail(XC_ERR_INVALID_ARGUMENT, "border frame: matrices too large to copy");
+        hipLaunchKernelGGL(frame_copy_kernel, dim3((uint32_t)tiles, (uint32_t)gy), dim3(256), 0, s, X, xn, xc, count,
+                           a.rows, a.cols, (uint32_t)tiles_c);
         XC_HIP(hipGetLastError());
-        return (const float*)Xt;
+        Xn = xn, Xc = xc;
     };
-    const float* Lt = interleave(L, nl);
-    const float* Rt = interleave(R, nr);
+    const float *Ln, *Lc, *Rn, *Rc;
+    copy(L, nl, Ln, Lc);
+    copy(R, nr, Rn, Rc);
     uint32_t lp_shift = 0;  // lanes per task: the power of two >= pairs, at most 64
     while (lp_shift < 6 && (1u << lp_shift) < a.pairs) ++lp_shift;
     const uint64_t lanes = (uint64_t)4 * total << lp_shift;
     const uint64_t chunks = ((uint64_t)a.pairs + (1u << lp_shift) - 1) >> lp_shift;
     if ((lanes + 255) / 256 > 0x7fffffffULL) fail(XC_ERR_INVALID_ARGUMENT, "border frame too large for one launch");
     dim3 grid((uint32_t)((lanes + 255) / 256), (uint32_t)std::min<uint64_t>(chunks, 65535));
+    // One task per wave with scalar addressing when 64 pairs share each task and the buffer
+    // loads' offsets (lane: 16 bytes per matrix; uniform: up to 3 quad positions) fit 31 bits.
+    const uint64_t most = std::max(nl, nr);
+    const bool wave_tasks = lp_shift == 6 && 16 * 4 * most < (1ull << 31);
     timed tf("frame_kernel", s);
-    hipLaunchKernelGGL(frame_kernel, grid, dim3(256), 0, s, Lt, Rt, O, a.family, a.rows, a.cols, a.n, a.m, nl, nr,
-                       a.pairs, thresh, lp_shift, tab);
+    if (wave_tasks)
+        hipLaunchKernelGGL(frame_kernel<true>, grid, dim3(256), 0, s, Ln, Rn, Lc, Rc, O, a.family, a.rows, a.cols, a.n,
+                           a.m, nl, nr, a.pairs, lp_shift, tab);
+    else
+        hipLaunchKernelGGL(frame_kernel<false>, grid, dim3(256), 0, s, Ln, Rn, Lc, Rc, O, a.family, a.rows, a.cols,
+                           a.n, a.m, nl, nr, a.pairs, lp_shift, tab);
     XC_HIP(hipGetLastError());
 }
 
