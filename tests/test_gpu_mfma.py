@@ -158,11 +158,15 @@ def test_mfma_tile_code_out_of_catalog_is_refused():
 @pytest.mark.parametrize("prec", ["bf16x3", "fp16x3"])
 @pytest.mark.parametrize("family,rows,cols,n,m,border", [
     ("one_to_one", 40, 70, 1, 1, 512), ("one_to_many", 30, 17, 1, 3, 100), ("n_to_mn", 25, 33, 3, 6, 300),
-    ("n_to_m", 20, 24, 9, 8, 512), ("n_to_m", 64, 9, 5, 27, 200), ("one_to_many", 300, 5, 1, 2, 512)])
+    ("n_to_m", 20, 24, 9, 8, 512), ("n_to_m", 64, 9, 5, 27, 200), ("one_to_many", 300, 5, 1, 2, 512),
+    ("one_to_many", 37, 21, 1, 70, 512), ("n_to_m", 19, 30, 2, 64, 300), ("n_to_mn", 13, 45, 1, 65, 512)])
 def test_border_frame_outputs_are_rounded_once(prec, family, rows, cols, n, m, border):
     """Outputs with overlap area below the border threshold are the exact sum rounded once to
     FP32: within half an FP32 ulp of the FP64 oracle.  Pair counts of 1 to 135 cover the task
-    grid's lanes-per-task choices (1 to 64 pairs per task, a partial last chunk)."""
+    grid's lanes-per-task choices (1 to 64 pairs per task, a partial last chunk); 70 pairs of
+    one left, n_to_m with 64 rights per left and n_to_mn with one left cover the tasks whose
+    64 pairs share their left matrix; tall (300 x 5) and wide (37 x 21) shapes put most of the
+    frame on the column-major or the row-major copies."""
     left = seeded((n, rows, cols), "u11", 31 + rows, np.float32)
     right = seeded((m, rows, cols), "u11", 37 + cols, np.float32)
     got = run(family, left, right, precision=prec, border=border).astype(np.float64)
