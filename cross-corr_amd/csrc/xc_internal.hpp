@@ -91,6 +91,18 @@ __device__ inline uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
     return (bid % XCD) * per + bid / XCD;
 }
 
+// The 16-byte quad at base + lane_bytes + uniform_bytes, where base and uniform_bytes are
+// the same for every lane of the wave: a raw buffer load (the base in a scalar resource, the
+// lane's offset in a VGPR, the uniform one in an SGPR), so the lanes need no address
+// arithmetic.  The caller keeps both offsets below 2^31.
+__device__ inline float4 buffer_quad(const float* base, uint32_t lane_bytes, uint32_t uniform_bytes) {
+    typedef int i32x4 __attribute__((ext_vector_type(4)));
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0, 0xffffffff, 0x00020000);
+    const i32x4 raw = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)lane_bytes, (int)uniform_bytes, 0);
+    return __builtin_bit_cast(float4, raw);
+}
+
 // Description of one launch over a regular family.
 struct family_args {
     int family;

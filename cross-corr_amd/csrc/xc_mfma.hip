@@ -23,6 +23,7 @@
 // Outputs whose overlap area is below the border threshold are recomputed exactly in FP32
 // by frame_kernel: there, few products are summed and the split error does not average out.
 #include <algorithm>
+#include <type_traits>
 
 #include "xc_methods.hpp"
 
@@ -939,19 +940,14 @@ __device__ __forceinline__ uint32_t frame_uniform(uint32_t v) {
     else return v;
 }
 
-// The quad at base + lane_bytes + uniform_bytes.  With a wave-uniform base (WAVE) this is a
-// raw buffer load: the base in a scalar resource, the lane's offset in a VGPR and the
-// uniform one in an SGPR, so no per-lane address arithmetic.  The host guarantees that the
-// offsets fit 31 bits.
+// The quad at base + lane_bytes + uniform_bytes: with a wave-uniform base (WAVE) a raw
+// buffer load (buffer_quad), else a plain per-lane load.  The host keeps the offsets below
+// 2^31.
 template <bool WAVE>
 __device__ __forceinline__ void frame_load4(const float* __restrict__ base, uint32_t lane_bytes,
                                             uint32_t uniform_bytes, float* v) {
     if constexpr (WAVE) {
-        const __amdgpu_buffer_rsrc_t rsrc =
-            __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0, 0xffffffff, 0x00020000);
-        typedef int i32x4 __attribute__((ext_vector_type(4)));
-        const i32x4 raw = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)lane_bytes, (int)uniform_bytes, 0);
-        const float4 q = __builtin_bit_cast(float4, raw);
+        const float4 q = buffer_quad(base, lane_bytes, uniform_bytes);
         v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
     } else {
         const float4 q =
@@ -960,7 +956,15 @@ __device__ __forceinline__ void frame_load4(const float* __restrict__ base, uint
     }
 }
 
-template <bool WAVE>
+// The same quad through the scalar cache, for an address that is uniform over the wave.
+__device__ __forceinline__ void frame_sload4(const float* __restrict__ p, float* v) {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
+}
+
+// MODE 0: a task per LP < 64 lanes; 1: a task per wave; 2: a task per wave whose 64 pairs
+// share one left matrix, whose quads are then scalar loads.
+template <int MODE>
 __global__ __launch_bounds__(256) void frame_kernel(const float* __restrict__ Ln, const float* __restrict__ Rn,
                                                     const float* __restrict__ Lc, const float* __restrict__ Rc,
                                                     float* __restrict__ O, int family, uint32_t rows,
@@ -968,6 +972,7 @@ __global__ __launch_bounds__(256) void frame_kernel(const float* __restrict__ Ln
                                                     uint32_t nr_mats, uint32_t pairs, uint32_t lp_shift,
                                                     frame_plan tab) {
     constexpr int G = kFrameGroup;
+    constexpr bool WAVE = MODE > 0;
     const uint32_t per_quad = tab.start[tab.ne];
     const uint32_t slot = frame_uniform<WAVE>((blockIdx.x * blockDim.x + threadIdx.x) >> lp_shift);
     if (slot >= 4 * per_quad) return;
@@ -1005,16 +1010,8 @@ __global__ __launch_bounds__(256) void frame_kernel(const float* __restrict__ Ln
         const bool valid = pk < pairs;  // lanes past the last pair run the last one and store nothing
         uint32_t li, ri;
         pair_of(family, n, m, valid ? pk : pairs - 1, li, ri);
+        if constexpr (MODE == 2) li = __builtin_amdgcn_readfirstlane(li);
         const uint32_t la = 16 * (a_left ? li : ri), lb = 16 * (a_left ? ri : li);  // lane byte offsets
-        // quads of step (i, v): A row ra+i from column s0 + G*v, B row rb+i from column G*v
-        auto fetch = [&](int row, int grp, float (&va)[2 * G], float (&vb)[G]) {
-            const float* __restrict__ ap = Ab + ((int64_t)(ra + row) * Qd + (s0 + G * grp) / 4) * sa * 4;
-            const float* __restrict__ bp = Bb + ((int64_t)(rb + row) * Qd + G * grp / 4) * sb * 4;
-#pragma unroll
-            for (int j = 0; j < G / 2; ++j) frame_load4<WAVE>(ap, la, (uint32_t)(16 * j * sa), va + 4 * j);
-#pragma unroll
-            for (int j = 0; j < G / 4; ++j) frame_load4<WAVE>(bp, lb, (uint32_t)(16 * j * sb), vb + 4 * j);
-        };
         double acc[G];
 #pragma unroll
         for (int k = 0; k < G; ++k) acc[k] = 0.0;
@@ -1029,24 +1026,49 @@ __global__ __launch_bounds__(256) void frame_kernel(const float* __restrict__ Ln
                 for (int k = 0; k < G; ++k) acc[k] = fma(da[j + k], bv, acc[k]);
             }
         };
-        // two register sets in turn: each step's successor is loaded while it is summed
-        float xa[2 * G], xb[G], ya[2 * G], yb[G];
-        int i = 0, v = 0;
-        auto next = [&]() {
-            if (++v == ng) {
-                v = 0;
-                ++i;
+        // SA, SB: A, B is the shared left matrix, read by scalar loads
+        auto run = [&](auto sa_c, auto sb_c) {
+            constexpr bool SA = decltype(sa_c)::value, SB = decltype(sb_c)::value;
+            // quads of step (row, grp): A row ra+row from column s0 + G*grp, B row rb+row from G*grp
+            auto fetch = [&](int row, int grp, float (&va)[2 * G], float (&vb)[G]) {
+                const float* __restrict__ ap = Ab + ((int64_t)(ra + row) * Qd + (s0 + G * grp) / 4) * sa * 4;
+                const float* __restrict__ bp = Bb + ((int64_t)(rb + row) * Qd + G * grp / 4) * sb * 4;
+#pragma unroll
+                for (int j = 0; j < G / 2; ++j) {
+                    if constexpr (SA) frame_sload4(ap + (j * sa + la / 16) * 4, va + 4 * j);
+                    else frame_load4<WAVE>(ap, la, (uint32_t)(16 * j * sa), va + 4 * j);
+                }
+#pragma unroll
+                for (int j = 0; j < G / 4; ++j) {
+                    if constexpr (SB) frame_sload4(bp + (j * sb + lb / 16) * 4, vb + 4 * j);
+                    else frame_load4<WAVE>(bp, lb, (uint32_t)(16 * j * sb), vb + 4 * j);
+                }
+            };
+            // two register sets in turn: each step's successor is loaded while it is summed
+            float xa[2 * G], xb[G], ya[2 * G], yb[G];
+            int i = 0, v = 0;
+            auto next = [&]() {
+                if (++v == ng) {
+                    v = 0;
+                    ++i;
+                }
+            };
+            fetch(0, 0, xa, xb);
+            for (int st = 0; st < nsteps; st += 2) {
+                next();
+                if (st + 1 < nsteps) fetch(i, v, ya, yb);
+                sum(xa, xb);
+                if (st + 1 == nsteps) break;
+                next();
+                if (st + 2 < nsteps) fetch(i, v, xa, xb);
+                sum(ya, yb);
             }
         };
-        fetch(0, 0, xa, xb);
-        for (int st = 0; st < nsteps; st += 2) {
-            next();
-            if (st + 1 < nsteps) fetch(i, v, ya, yb);
-            sum(xa, xb);
-            if (st + 1 == nsteps) break;
-            next();
-            if (st + 2 < nsteps) fetch(i, v, xa, xb);
-            sum(ya, yb);
+        if constexpr (MODE == 2) {
+            if (a_left) run(std::true_type{}, std::false_type{});
+            else run(std::false_type{}, std::true_type{});
+        } else {
+            run(std::false_type{}, std::false_type{});
         }
         if (valid) {
             float* __restrict__ op = O + (size_t)pk * SX * SY;
@@ -1169,13 +1191,17 @@ void launch_frame_f32(const family_args& a, const float* L, const float* R, floa
     // loads' offsets (lane: 16 bytes per matrix; uniform: up to 3 quad positions) fit 31 bits.
     const uint64_t most = std::max(nl, nr);
     const bool wave_tasks = lp_shift == 6 && 16 * 4 * most < (1ull << 31);
+    // The 64 pairs of a wave share their left matrix (pairs k = 64c .. 64c+63).
+    const bool shared_left = a.family == XC_ONE_TO_MANY || (a.family == XC_N_TO_M && a.m % 64 == 0) ||
+                             (a.family == XC_N_TO_MN && a.n == 1);
     timed tf("frame_kernel", s);
-    if (wave_tasks)
-        hipLaunchKernelGGL(frame_kernel<true>, grid, dim3(256), 0, s, Ln, Rn, Lc, Rc, O, a.family, a.rows, a.cols, a.n,
-                           a.m, nl, nr, a.pairs, lp_shift, tab);
-    else
-        hipLaunchKernelGGL(frame_kernel<false>, grid, dim3(256), 0, s, Ln, Rn, Lc, Rc, O, a.family, a.rows, a.cols,
-                           a.n, a.m, nl, nr, a.pairs, lp_shift, tab);
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, dim3(256), 0, s, Ln, Rn, Lc, Rc, O, a.family, a.rows, a.cols, a.n, a.m, nl, nr,
+                           a.pairs, lp_shift, tab);
+    };
+    if (wave_tasks && shared_left) go(frame_kernel<2>);
+    else if (wave_tasks) go(frame_kernel<1>);
+    else go(frame_kernel<0>);
     XC_HIP(hipGetLastError());
 }
 
