@@ -133,9 +133,253 @@ direct_plan plan_direct(const family_args& a, const xc_options& opt) {
     return d;
 }
 
+// ---------------------------------------------------------------- lanes over pairs
+// Definition-based FP32 kernel for batches whose 64 consecutive pairs share their left
+// matrix (one_to_many; n_to_m with m a multiple of 64; n_to_mn with one left).  A wave takes
+// 64 pairs, one per lane, and a tile of TY consecutive dy x 4 consecutive dx, so every lane
+// runs the same loops: no lane spends FMAs on another lane's zero padding (the register-tile
+// kernel's lanes take 64 consecutive dx and share the union of their column ranges), and
+// the left values are wave-uniform scalar loads.  Per right row j and chunk of 4 columns
+// c0 .. c0+3 a lane loads one quad of its right row and does TY*4*4 FMAs
+//   out(dy0+y, dx0+x) += L[j+dy0+y][c0+t] * R[j][c0+t-dx0-x],   y < TY, x, t < 4,
+// the right values R[j][c0-dx0-3 .. c0-dx0+4] being a window of two quads that slides by one
+// quad per chunk.  The inputs come from padded copies: the rights as quads interleaved over
+// the matrices (quad position p of row j of matrix i at ((j * Q4 + p) * count + i) * 4, column
+// q at position q + PL, PL = 8 + (-W mod 4), so every window starts on a quad and reads zeros
+// outside [0, W)); the lefts row-major with TY-1 zero rows above and below and rows padded with
+// zeros to a multiple of 4.  Products are summed in FP32 partials of 64 that are added into
+// FP64 totals.  Tiles run nearest-the-centre first (the largest overlaps start first).
+struct dp_args {
+    int family, H, W;
+    uint32_t n, m, pairs, groups;     // groups of 64 pairs
+    uint32_t tiles_y, tiles_x, nbx;   // nbx: blocks of 4 x-tiles (one per wave)
+    uint32_t r_count;                 // matrices in the right copy
+    int Q4, PL, Hl, Wl;               // right copy: quads per row, column offset; left copy: rows, row length
+    int by_unit;                      // block order (see the kernel)
+};
+
+__device__ inline int center_first(int k, int count) {
+    const int c = (count - 1) / 2;
+    return (k & 1) ? c + (k + 1) / 2 : c - k / 2;
+}
+
+template <int TY>
+__global__ __launch_bounds__(256) void direct_pairs_kernel(const float* __restrict__ Lc, const float* __restrict__ Rq,
+                                                           float* __restrict__ O, dp_args a) {
+    constexpr int TX = 4, FLUSHC = 16;  // FLUSHC chunks of 4 columns per FP32 partial
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    // Blocks go round-robin to the 8 XCDs; tiles run nearest the centre (largest overlap)
+    // first.  Right copies larger than an XCD's L2 (by_unit): a unit (one x-block of 4
+    // tiles, one group of 64 pairs) keeps all its dy bands on one XCD; those waves have the
+    // same column range, so they take about the same time per right row and read each row
+    // while it is in that XCD's L2.  Units are dealt to the XCDs in snake order.  Smaller
+    // copies: a band of TY dy (all its x-blocks and groups) per XCD, bands dealt in turn
+    // (measured faster where the copy fits the L2).
+    const uint32_t xcd = blockIdx.x % 8, pos = blockIdx.x / 8;
+    uint32_t g;
+    int ty, kx;
+    if (a.by_unit) {
+        const uint32_t round = pos / a.tiles_y;
+        const uint32_t u = round * 8 + ((round & 1) ? 7 - xcd : xcd);
+        if (u >= a.groups * a.nbx) return;
+        g = u % a.groups;
+        kx = center_first((int)(u / a.groups), (int)a.nbx);
+        ty = center_first((int)(pos % a.tiles_y), (int)a.tiles_y);
+    } else {
+        const uint32_t per_band = a.groups * a.nbx;
+        const uint32_t band = pos / per_band * 8 + xcd;
+        if (band >= a.tiles_y) return;
+        g = (pos % per_band) % a.groups;
+        kx = center_first((int)((pos % per_band) / a.groups), (int)a.nbx);
+        ty = center_first((int)band, (int)a.tiles_y);
+    }
+    const int tx = kx * 4 + wave;
+    if (tx >= (int)a.tiles_x) return;
+    const int H = a.H, W = a.W;
+    const int dy0 = -(H - 1) + TY * ty, dx0 = -(W - 1) + TX * tx;
+    const uint32_t pk = g * 64 + (uint32_t)lane;
+    const bool valid = pk < a.pairs;  // lanes past the last pair run the last one and store nothing
+    uint32_t li, ri;
+    pair_of(a.family, a.n, a.m, valid ? pk : a.pairs - 1, li, ri);
+    li = __builtin_amdgcn_readfirstlane(li);  // the same for the wave's pairs
+    // right rows: those that meet a dy of the tile; columns: the union over the tile's dx,
+    // from a multiple of 4
+    const int jlo = max(0, -(dy0 + TY - 1)), jhi = min(H, H - dy0);
+    const int clo = max(0, dx0) & ~3;
+    const int nch = (min(W, W + dx0 + TX - 1) - clo + 3) >> 2;
+    const uint32_t lane_bytes = ri * 16u;
+    const int64_t rq = (int64_t)a.r_count * 4;  // floats between quad positions of one row
+    const float* __restrict__ lt = Lc + ((int64_t)li * a.Hl + dy0 + TY - 1) * a.Wl + clo;  // + j * Wl: row j + dy0
+    const float* __restrict__ rt = Rq + (int64_t)((clo - dx0 - 3 + a.PL) >> 2) * rq;       // + j * Q4 * rq: row j
+    float acc[TY][TX];
+    double tot[TY][TX];
+#pragma unroll
+    for (int y = 0; y < TY; ++y)
+#pragma unroll
+        for (int x = 0; x < TX; ++x) acc[y][x] = 0.f, tot[y][x] = 0.0;
+    int cnt = 0;
+    auto flush = [&]() {
+#pragma unroll
+        for (int y = 0; y < TY; ++y)
+#pragma unroll
+            for (int x = 0; x < TX; ++x) {
+                tot[y][x] += (double)acc[y][x];
+                acc[y][x] = 0.f;
+            }
+    };
+    for (int j = jlo; j < jhi; ++j) {
+        const float* __restrict__ lr = lt + (int64_t)j * a.Wl;
+        const float* __restrict__ rr = rt + (int64_t)j * a.Q4 * rq;
+        float4 w0 = buffer_quad(rr, lane_bytes, 0), w1 = buffer_quad(rr + rq, lane_bytes, 0);
+        float4 lv[TY];
+#pragma unroll
+        for (int y = 0; y < TY; ++y) lv[y] = *reinterpret_cast<const float4*>(lr + (int64_t)y * a.Wl);
+        for (int ch = 0; ch < nch; ++ch) {
+            // the next chunk's values in flight during this one (the copies hold a quad past
+            // the last window, the left copy a chunk past the last one)
+            const float4 w2 = buffer_quad(rr + (ch + 2) * rq, lane_bytes, 0);
+            float4 ln[TY];
+#pragma unroll
+            for (int y = 0; y < TY; ++y) ln[y] = *reinterpret_cast<const float4*>(lr + (int64_t)y * a.Wl + 4 * ch + 4);
+            const float w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+            for (int y = 0; y < TY; ++y) {
+                const float l[4] = {lv[y].x, lv[y].y, lv[y].z, lv[y].w};
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int x = 0; x < TX; ++x) acc[y][x] = fmaf(l[t], w[t + 3 - x], acc[y][x]);
+            }
+            w0 = w1;
+            w1 = w2;
+#pragma unroll
+            for (int y = 0; y < TY; ++y) lv[y] = ln[y];
+            if (++cnt == FLUSHC) {
+                flush();
+                cnt = 0;
+            }
+        }
+    }
+    flush();
+    if (!valid) return;
+    float* __restrict__ op = O + (size_t)pk * (2 * W - 1) * (2 * H - 1);
+#pragma unroll
+    for (int y = 0; y < TY; ++y) {
+        const int dy = dy0 + y;
+        if (dy > H - 1) break;
+#pragma unroll
+        for (int x = 0; x < TX; ++x)
+            if (dx0 + x <= W - 1) op[(size_t)(dy + H - 1) * (2 * W - 1) + (dx0 + x + W - 1)] = (float)tot[y][x];
+    }
+}
+
+// Left copy: count matrices of H x W into rows of Wl (zero past W) with pad zero rows above
+// and below: element (r, c) of matrix i at (i * Hl + r + pad) * Wl + c.
+__global__ __launch_bounds__(256) void pairs_copy_left_kernel(const float* __restrict__ X, float* __restrict__ Xc,
+                                                              uint32_t count, int H, int W, int Hl, int Wl, int pad) {
+    const uint64_t total = (uint64_t)count * Hl * Wl;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(e % (uint64_t)Wl);
+        const uint64_t rest = e / (uint64_t)Wl;
+        const int r = (int)(rest % (uint64_t)Hl) - pad;
+        const uint64_t i = rest / (uint64_t)Hl;
+        Xc[e] = (r >= 0 && r < H && c < W) ? X[(i * H + r) * W + c] : 0.f;
+    }
+}
+
+// Right copy: quad position p of row j of matrix i at ((j * Q4 + p) * count + i) * 4, holding
+// columns 4p - PL .. 4p - PL + 3 (zero outside [0, W)).  Thread e writes quad e.
+__global__ __launch_bounds__(256) void pairs_copy_right_kernel(const float* __restrict__ X, float* __restrict__ Xq,
+                                                               uint32_t count, int H, int W, int Q4, int PL) {
+    const uint64_t total = (uint64_t)count * H * Q4;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = e % count, rest = e / count;
+        const int p = (int)(rest % (uint64_t)Q4), j = (int)(rest / (uint64_t)Q4);
+        const float* __restrict__ row = X + (i * H + j) * W;
+        float v[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int q = 4 * p + t - PL;
+            v[t] = (q >= 0 && q < W) ? row[q] : 0.f;
+        }
+        *reinterpret_cast<float4*>(Xq + e * 4) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+}
+
+constexpr int kPairsTY = 4;
+
+// Whether the lanes-over-pairs kernel serves this batch: 64 consecutive pairs share a left.
+bool pairs_kernel_applies(const family_args& a) {
+    const bool shared_left = a.family == XC_ONE_TO_MANY || (a.family == XC_N_TO_M && a.m % 64 == 0) ||
+                             (a.family == XC_N_TO_MN && a.n == 1);
+    return shared_left && a.pairs >= 64;
+}
+
+// Waves of the lanes-over-pairs kernel: below 4 per SIMD it cannot hide its load latency.
+uint64_t pairs_kernel_waves(const family_args& a) {
+    return (uint64_t)((a.pairs + 63) / 64) * ((2 * a.rows - 1 + kPairsTY - 1) / kPairsTY) * ((2 * a.cols - 1 + 3) / 4);
+}
+
+void launch_direct_pairs(const family_args& a, const float* L, const float* R, float* O, hipStream_t s) {
+    constexpr int TY = kPairsTY;
+    const int H = (int)a.rows, W = (int)a.cols;
+    const uint32_t nl = a.family == XC_N_TO_M ? a.n : 1;
+    const uint32_t nr = a.m;
+    dp_args k{};
+    k.family = a.family;
+    k.H = H;
+    k.W = W;
+    k.n = a.family == XC_ONE_TO_MANY ? 1 : a.n;
+    k.m = a.m;
+    k.pairs = a.pairs;
+    k.groups = (a.pairs + 63) / 64;
+    k.tiles_y = (uint32_t)(2 * H - 1 + TY - 1) / TY;
+    k.tiles_x = (uint32_t)(2 * W - 1 + 3) / 4;
+    k.nbx = (k.tiles_x + 3) / 4;
+    k.r_count = nr;
+    k.PL = 8 + (4 - W % 4) % 4;
+    k.Q4 = (k.PL + W + 12 + 3) / 4;  // a quad past the last window
+    k.Hl = H + 2 * (TY - 1);
+    k.Wl = (W + 3) / 4 * 4 + 4;       // a chunk past the last one (prefetch)
+    if ((uint64_t)nr * 16 >= (1ull << 31)) fail(XC_ERR_UNSUPPORTED, "direct kernel: too many right matrices");
+    const uint64_t lsize = (uint64_t)nl * k.Hl * k.Wl, rsize = (uint64_t)nr * H * k.Q4 * 4;
+    float* Lc = static_cast<float*>(workspace(WS_DIRECT, sizeof(float) * (lsize + rsize), s));
+    float* Rq = Lc + lsize;
+    auto blocks_for = [](uint64_t n) { return (uint32_t)std::min<uint64_t>((n + 255) / 256, 16384); };
+    hipLaunchKernelGGL(pairs_copy_left_kernel, dim3(blocks_for(lsize)), dim3(256), 0, s, L, Lc, nl, H, W, k.Hl, k.Wl,
+                       TY - 1);
+    XC_HIP(hipGetLastError());
+    hipLaunchKernelGGL(pairs_copy_right_kernel, dim3(blocks_for(rsize / 4)), dim3(256), 0, s, R, Rq, nr, H, W, k.Q4,
+                       k.PL);
+    XC_HIP(hipGetLastError());
+    k.by_unit = rsize * sizeof(float) > (4u << 20);  // larger than one XCD's L2
+    const uint64_t nblocks = k.by_unit ? ((uint64_t)k.groups * k.nbx + 7) / 8 * 8 * k.tiles_y
+                                       : (uint64_t)k.groups * k.nbx * ((k.tiles_y + 7) / 8 * 8);
+    if (nblocks > 0x7fffffffULL) fail(XC_ERR_INVALID_ARGUMENT, "problem too large for one launch");
+    dim3 grid((uint32_t)nblocks), block(256);
+    record_launch("direct_pairs_kernel<f32> ty4 tx4", grid, block, 0, a.rows, a.cols, a.pairs, 1);
+    timed td("direct_kernel", s);
+    hipLaunchKernelGGL(direct_pairs_kernel<TY>, grid, block, 0, s, Lc, Rq, O, k);
+    XC_HIP(hipGetLastError());
+}
+
 template <typename T>
 void launch_direct(const family_args& a, const T* L, const T* R, T* O, const xc_options& opt, hipStream_t s) {
     if (a.cols > (uint32_t)kMaxCols) fail(XC_ERR_UNSUPPORTED, "direct kernel: cols above 32768");
+    if constexpr (sizeof(T) == 4) {
+        // tile 0: the lanes-over-pairs kernel where it applies and no register tile or work
+        // distribution is requested; 1: the register-tile kernel; 2: the lanes-over-pairs kernel
+        if (opt.tile > 2) fail(XC_ERR_INVALID_ARGUMENT, "FP32 tile must be 0 (default), 1 (register tiles) or 2 (lanes over pairs)");
+        const bool keys = opt.shifts_per_thread || opt.right_mats_per_thread || opt.left_mats_per_thread ||
+                          opt.rows_per_step || opt.waves_per_block || opt.split || opt.rows_per_block;
+        if (opt.tile == 2 && !pairs_kernel_applies(a))
+            fail(XC_ERR_UNSUPPORTED, "FP32 tile 2 needs 64 or more pairs whose consecutive 64 share a left matrix");
+        if (opt.tile == 2 || (opt.tile == 0 && !keys && pairs_kernel_applies(a) && pairs_kernel_waves(a) >= 4096)) {
+            launch_direct_pairs(a, L, R, O, s);
+            return;
+        }
+    }
     direct_plan d = plan_direct<T>(a, opt);
     if (d.lds > 160 * 1024) {
         if (opt.rows_per_step || opt.right_mats_per_thread)

@@ -17,7 +17,7 @@ void norm_error_f32(const float* x, const double* ref, const double* den, uint64
 void norm_error_f64(const double* x, const double* ref, const double* den, uint64_t n, double* out3, hipStream_t s);
 enum : int {
     WS_FFT_PAD = 0, WS_FFT_SPEC = 1, WS_FFT_OUT = 2, WS_FFT_RES = 3, WS_SPLIT = 4, WS_BATCH = 5,
-    WS_FK_SR = 6, WS_FK_SL = 7, WS_FK_LHAT = 8, WS_FK_Z = 9, WS_FRAME = 10, WS_CHECK = 11
+    WS_FK_SR = 6, WS_FK_SL = 7, WS_FK_LHAT = 8, WS_FK_Z = 9, WS_FRAME = 10, WS_CHECK = 11, WS_DIRECT = 12
 };
 
 // Dispatch of xc_cross_corr_f32 on opt.precision (xc_precision).

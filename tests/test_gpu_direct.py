@@ -160,6 +160,46 @@ def test_direct_triangle_distribution(rows_per_block):
             check("n_to_m", left, right, got, tol)
 
 
+# The lanes-over-pairs kernel (FP32 tile 2, and the default where every 64 consecutive pairs
+# share their left matrix and the batch makes enough waves): one pair per lane, 64 pairs per
+# wave, a partial last group of pairs, outputs narrower and wider than a tile, one row or
+# column.
+PAIRS_SHAPES = [("one_to_many", 1, 1, 1, 64), ("one_to_many", 7, 5, 1, 70), ("one_to_many", 33, 65, 1, 128),
+                ("n_to_m", 17, 23, 2, 64), ("n_to_mn", 12, 30, 1, 65), ("one_to_many", 3, 200, 1, 64),
+                ("one_to_many", 150, 9, 1, 64), ("one_to_many", 1, 37, 1, 66), ("n_to_m", 40, 1, 3, 128)]
+
+
+@pytest.mark.parametrize("family,rows,cols,n,m", PAIRS_SHAPES)
+def test_direct_lanes_over_pairs(family, rows, cols, n, m):
+    x = xcorr()
+    left = seeded((n, rows, cols), "u11", 900 + rows, np.float32)
+    right = seeded((m, rows, cols), "u11", 950 + cols, np.float32)
+    print(f"lanes over pairs {family} {rows}x{cols} n={n} m={m}", flush=True)
+    got = run(family, left, right, precision="fp32", tile=2)
+    assert x.last_launch()["kernel"].startswith("direct_pairs_kernel")
+    check(family, left, right, got, 1e-6)
+    default = run(family, left, right, precision="fp32")  # either kernel, by the batch's size
+    check(family, left, right, default, 1e-6)
+    tiles = run(family, left, right, precision="fp32", tile=1)
+    assert x.last_launch()["kernel"].startswith("direct_kernel")
+    check(family, left, right, tiles, 1e-6)
+
+
+def test_direct_lanes_over_pairs_refused_where_it_does_not_apply():
+    x = xcorr()
+    for family, n, m in (("one_to_many", 1, 63), ("n_to_m", 2, 32), ("n_to_mn", 2, 128), ("one_to_one", 1, 1)):
+        left = dev(seeded((n, 9, 9), "u01", 1, np.float32))
+        right = dev(seeded((m, 9, 9), "u01", 2, np.float32))
+        with pytest.raises(x.XcError) as e:
+            x.cross_corr(family, left, right, precision="fp32", tile=2)
+        assert e.value.status == 4
+        x.cross_corr(family, left, right, precision="fp32")  # the register-tile kernel serves it
+        assert x.last_launch()["kernel"].startswith("direct_kernel")
+    with pytest.raises(x.XcError) as e:
+        x.cross_corr("one_to_one", left, right, precision="fp32", tile=3)
+    assert e.value.status == 1
+
+
 def test_direct_split_and_triangle_are_exclusive():
     x = xcorr()
     left = dev(seeded((1, 8, 8), "u01", 1, np.float32))
