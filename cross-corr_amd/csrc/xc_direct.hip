@@ -316,7 +316,8 @@ bool pairs_kernel_applies(const family_args& a) {
     return shared_left && a.pairs >= 64;
 }
 
-// Waves of the lanes-over-pairs kernel: below 4 per SIMD it cannot hide its load latency.
+// Waves of the lanes-over-pairs kernel.  Measured against the register-tile kernel (one_to_many
+// and n_to_m, 16^2 to 256^2): 0.45-0.9x at 128-1024 waves, 1.14x at 2048, 1.6-2.2x from 4096.
 uint64_t pairs_kernel_waves(const family_args& a) {
     return (uint64_t)((a.pairs + 63) / 64) * ((2 * a.rows - 1 + kPairsTY - 1) / kPairsTY) * ((2 * a.cols - 1 + 3) / 4);
 }
@@ -368,14 +369,15 @@ template <typename T>
 void launch_direct(const family_args& a, const T* L, const T* R, T* O, const xc_options& opt, hipStream_t s) {
     if (a.cols > (uint32_t)kMaxCols) fail(XC_ERR_UNSUPPORTED, "direct kernel: cols above 32768");
     if constexpr (sizeof(T) == 4) {
-        // tile 0: the lanes-over-pairs kernel where it applies and no register tile or work
-        // distribution is requested; 1: the register-tile kernel; 2: the lanes-over-pairs kernel
+        // tile 0: the lanes-over-pairs kernel where it applies, the batch gives 2048 waves or
+        // more and no register tile or work distribution is requested; 1: the register-tile
+        // kernel; 2: the lanes-over-pairs kernel
         if (opt.tile > 2) fail(XC_ERR_INVALID_ARGUMENT, "FP32 tile must be 0 (default), 1 (register tiles) or 2 (lanes over pairs)");
         const bool keys = opt.shifts_per_thread || opt.right_mats_per_thread || opt.left_mats_per_thread ||
                           opt.rows_per_step || opt.waves_per_block || opt.split || opt.rows_per_block;
         if (opt.tile == 2 && !pairs_kernel_applies(a))
             fail(XC_ERR_UNSUPPORTED, "FP32 tile 2 needs 64 or more pairs whose consecutive 64 share a left matrix");
-        if (opt.tile == 2 || (opt.tile == 0 && !keys && pairs_kernel_applies(a) && pairs_kernel_waves(a) >= 4096)) {
+        if (opt.tile == 2 || (opt.tile == 0 && !keys && pairs_kernel_applies(a) && pairs_kernel_waves(a) >= 2048)) {
             launch_direct_pairs(a, L, R, O, s);
             return;
         }

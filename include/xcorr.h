@@ -87,7 +87,9 @@ typedef enum xc_precision {
 /* Optional tuning; zero-initialise and set what you need (0 = library default). */
 typedef struct xc_options {
     uint32_t precision;   /* xc_precision */
-    uint32_t tile;        /* kernel-specific tile code from the tile catalog, 0 = default */
+    uint32_t tile;        /* kernel-specific tile code from the tile catalog, 0 = default;
+                             FP32 (float data): 1 = register-tile kernel, 2 = lanes-over-pairs
+                             kernel (64 consecutive pairs sharing a left matrix) */
     uint32_t split;       /* FP32: split of the reduction (overlap rows) across blocks;
                              FFT64 tile 1: at most this many pairs per launch of the column
                              and inverse-row kernels (less scratch memory); 0 = auto */
