@@ -158,11 +158,6 @@ struct dp_args {
     int by_unit;                      // block order (see the kernel)
 };
 
-__device__ inline int center_first(int k, int count) {
-    const int c = (count - 1) / 2;
-    return (k & 1) ? c + (k + 1) / 2 : c - k / 2;
-}
-
 template <int TY>
 __global__ __launch_bounds__(256) void direct_pairs_kernel(const float* __restrict__ Lc, const float* __restrict__ Rq,
                                                            float* __restrict__ O, dp_args a) {
@@ -176,25 +171,11 @@ __global__ __launch_bounds__(256) void direct_pairs_kernel(const float* __restri
     // while it is in that XCD's L2.  Units are dealt to the XCDs in snake order.  Smaller
     // copies: a band of TY dy (all its x-blocks and groups) per XCD, bands dealt in turn
     // (measured faster where the copy fits the L2).
-    const uint32_t xcd = blockIdx.x % 8, pos = blockIdx.x / 8;
-    uint32_t g;
-    int ty, kx;
-    if (a.by_unit) {
-        const uint32_t round = pos / a.tiles_y;
-        const uint32_t u = round * 8 + ((round & 1) ? 7 - xcd : xcd);
-        if (u >= a.groups * a.nbx) return;
-        g = u % a.groups;
-        kx = center_first((int)(u / a.groups), (int)a.nbx);
-        ty = center_first((int)(pos % a.tiles_y), (int)a.tiles_y);
-    } else {
-        const uint32_t per_band = a.groups * a.nbx;
-        const uint32_t band = pos / per_band * 8 + xcd;
-        if (band >= a.tiles_y) return;
-        g = (pos % per_band) % a.groups;
-        kx = center_first((int)((pos % per_band) / a.groups), (int)a.nbx);
-        ty = center_first((int)band, (int)a.tiles_y);
-    }
-    const int tx = kx * 4 + wave;
+    uint32_t u, band;
+    if (!block_order(blockIdx.x, a.groups * a.nbx, a.tiles_y, a.by_unit, u, band)) return;
+    const uint32_t g = u % a.groups;
+    const int ty = center_first((int)band, (int)a.tiles_y);
+    const int tx = center_first((int)(u / a.groups), (int)a.nbx) * 4 + wave;
     if (tx >= (int)a.tiles_x) return;
     const int H = a.H, W = a.W;
     const int dy0 = -(H - 1) + TY * ty, dx0 = -(W - 1) + TX * tx;
@@ -355,8 +336,7 @@ void launch_direct_pairs(const family_args& a, const float* L, const float* R, f
                        k.PL);
     XC_HIP(hipGetLastError());
     k.by_unit = rsize * sizeof(float) > (4u << 20);  // larger than one XCD's L2
-    const uint64_t nblocks = k.by_unit ? ((uint64_t)k.groups * k.nbx + 7) / 8 * 8 * k.tiles_y
-                                       : (uint64_t)k.groups * k.nbx * ((k.tiles_y + 7) / 8 * 8);
+    const uint64_t nblocks = blocks_for_order(k.groups * k.nbx, k.tiles_y, k.by_unit);
     if (nblocks > 0x7fffffffULL) fail(XC_ERR_INVALID_ARGUMENT, "problem too large for one launch");
     dim3 grid((uint32_t)nblocks), block(256);
     record_launch("direct_pairs_kernel<f32> ty4 tx4", grid, block, 0, a.rows, a.cols, a.pairs, 1);

@@ -103,6 +103,34 @@ __device__ inline float4 buffer_quad(const float* base, uint32_t lane_bytes, uin
     return __builtin_bit_cast(float4, raw);
 }
 
+// Block order for kernels whose blocks form a grid of units x bands (a unit: the work that
+// shares one column range and one group of pairs; a band: one range of dy), dispatched
+// round-robin over the 8 XCDs.  by_unit: all bands of a unit on one XCD, units dealt to the
+// XCDs in snake order; else a band (all its units) per XCD, bands dealt in turn.  The grid
+// has blocks_for_order() blocks; a block maps to (unit, band rank) or to nothing (false).
+__host__ __device__ inline uint64_t blocks_for_order(uint32_t units, uint32_t bands, bool by_unit) {
+    return by_unit ? ((uint64_t)units + 7) / 8 * 8 * bands : (uint64_t)units * ((bands + 7) / 8 * 8);
+}
+__device__ inline bool block_order(uint32_t bid, uint32_t units, uint32_t bands, bool by_unit, uint32_t& unit,
+                                   uint32_t& band) {
+    const uint32_t xcd = bid % 8, pos = bid / 8;
+    if (by_unit) {
+        const uint32_t round = pos / bands;
+        unit = round * 8 + ((round & 1) ? 7 - xcd : xcd);
+        band = pos % bands;
+        return unit < units;
+    }
+    band = pos / units * 8 + xcd;
+    unit = pos % units;
+    return band < bands;
+}
+
+// Rank k of count in nearest-the-centre-first order: c, c+1, c-1, c+2, ... (c = (count-1)/2).
+__host__ __device__ inline int center_first(int k, int count) {
+    const int c = (count - 1) / 2;
+    return (k & 1) ? c + (k + 1) / 2 : c - k / 2;
+}
+
 // Description of one launch over a regular family.
 struct family_args {
     int family;
