@@ -134,9 +134,10 @@ direct_plan plan_direct(const family_args& a, const xc_options& opt) {
 }
 
 // ---------------------------------------------------------------- lanes over pairs
-// Definition-based FP32 kernel for batches whose 64 consecutive pairs share their left
-// matrix (one_to_many; n_to_m with m a multiple of 64; n_to_mn with one left).  A wave takes
-// 64 pairs, one per lane, and a tile of TY consecutive dy x 4 consecutive dx, so every lane
+// Definition-based FP32 kernel for batches with many rights per left (one_to_many, n_to_m,
+// n_to_mn).  A wave takes one left and 64 of its rights (the pairs (left, right), one per
+// lane; a left's last wave may be partial) and a tile of TY consecutive dy x 4 consecutive
+// dx, so every lane
 // runs the same loops: no lane spends FMAs on another lane's zero padding (the register-tile
 // kernel's lanes take 64 consecutive dx and share the union of their column ranges), and
 // the left values are wave-uniform scalar loads.  Per right row j and chunk of 4 columns
@@ -144,14 +145,17 @@ direct_plan plan_direct(const family_args& a, const xc_options& opt) {
 //   out(dy0+y, dx0+x) += L[j+dy0+y][c0+t] * R[j][c0+t-dx0-x],   y < TY, x, t < 4,
 // the right values R[j][c0-dx0-3 .. c0-dx0+4] being a window of two quads that slides by one
 // quad per chunk.  The inputs come from padded copies: the rights as quads interleaved over
-// the matrices (quad position p of row j of matrix i at ((j * Q4 + p) * count + i) * 4, column
-// q at position q + PL, PL = 8 + (-W mod 4), so every window starts on a quad and reads zeros
-// outside [0, W)); the lefts row-major with TY-1 zero rows above and below and rows padded with
-// zeros to a multiple of 4.  Products are summed in FP32 partials of 64 that are added into
+// slots (quad position p of row j of slot i at ((j * Q4 + p) * count + i) * 4, column q at
+// position q + PL, PL = 8 + (-W mod 4), so every window starts on a quad and reads zeros
+// outside [0, W)), the slots ordered so that a wave's 64 rights are consecutive (n_to_mn:
+// left l's rights l, l+n, l+2n, ... at slots l*MP .. l*MP+MP-1, MP their count rounded up to
+// 64; zero past the last); the lefts row-major with TY-1 zero rows above and below and rows
+// padded with zeros to a multiple of 4.  Products are summed in FP32 partials of 64 that are added into
 // FP64 totals.  Tiles run nearest-the-centre first (the largest overlaps start first).
 struct dp_args {
     int family, H, W;
-    uint32_t n, m, pairs, groups;     // groups of 64 pairs
+    uint32_t n, m, pairs, groups;     // groups: waves of 64 rights of one left, all lefts
+    uint32_t mpl, wpl;                // rights per left, waves per left
     uint32_t tiles_y, tiles_x, nbx;   // nbx: blocks of 4 x-tiles (one per wave)
     uint32_t r_count;                 // matrices in the right copy
     int Q4, PL, Hl, Wl;               // right copy: quads per row, column offset; left copy: rows, row length
@@ -179,17 +183,18 @@ __global__ __launch_bounds__(256) void direct_pairs_kernel(const float* __restri
     if (tx >= (int)a.tiles_x) return;
     const int H = a.H, W = a.W;
     const int dy0 = -(H - 1) + TY * ty, dx0 = -(W - 1) + TX * tx;
-    const uint32_t pk = g * 64 + (uint32_t)lane;
-    const bool valid = pk < a.pairs;  // lanes past the last pair run the last one and store nothing
-    uint32_t li, ri;
-    pair_of(a.family, a.n, a.m, valid ? pk : a.pairs - 1, li, ri);
-    li = __builtin_amdgcn_readfirstlane(li);  // the same for the wave's pairs
+    // the wave's left and its lanes' rights (t: index among the left's rights; lanes past the
+    // last right read a zero slot and store nothing)
+    const uint32_t li = g / a.wpl, t = (g % a.wpl) * 64 + (uint32_t)lane;
+    const bool valid = t < a.mpl;
+    const uint32_t slot = (a.family == XC_N_TO_MN ? li * a.wpl * 64 : 0) + t;
+    const uint32_t pk = a.family == XC_N_TO_M ? li * a.m + t : (a.family == XC_N_TO_MN ? li + a.n * t : t);
     // right rows: those that meet a dy of the tile; columns: the union over the tile's dx,
     // from a multiple of 4
     const int jlo = max(0, -(dy0 + TY - 1)), jhi = min(H, H - dy0);
     const int clo = max(0, dx0) & ~3;
     const int nch = (min(W, W + dx0 + TX - 1) - clo + 3) >> 2;
-    const uint32_t lane_bytes = ri * 16u;
+    const uint32_t lane_bytes = slot * 16u;
     const int64_t rq = (int64_t)a.r_count * 4;  // floats between quad positions of one row
     const float* __restrict__ lt = Lc + ((int64_t)li * a.Hl + dy0 + TY - 1) * a.Wl + clo;  // + j * Wl: row j + dy0
     const float* __restrict__ rt = Rq + (int64_t)((clo - dx0 - 3 + a.PL) >> 2) * rq;       // + j * Q4 * rq: row j
@@ -269,20 +274,30 @@ __global__ __launch_bounds__(256) void pairs_copy_left_kernel(const float* __res
     }
 }
 
-// Right copy: quad position p of row j of matrix i at ((j * Q4 + p) * count + i) * 4, holding
-// columns 4p - PL .. 4p - PL + 3 (zero outside [0, W)).  Thread e writes quad e.
+// Right copy: quad position p of row j of slot i at ((j * Q4 + p) * count + i) * 4, holding
+// columns 4p - PL .. 4p - PL + 3 (zero outside [0, W)) of the slot's matrix: with ln = 0 slot
+// i is matrix i (zero for i >= nmat); with ln lefts (n_to_mn), slot l*MP + t is matrix
+// l + ln*t (zero for t >= mpl).  Thread e writes quad e.
 __global__ __launch_bounds__(256) void pairs_copy_right_kernel(const float* __restrict__ X, float* __restrict__ Xq,
-                                                               uint32_t count, int H, int W, int Q4, int PL) {
+                                                               uint32_t count, uint32_t nmat, uint32_t ln,
+                                                               uint32_t mp, uint32_t mpl, int H, int W, int Q4, int PL) {
     const uint64_t total = (uint64_t)count * H * Q4;
     for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t i = e % count, rest = e / count;
         const int p = (int)(rest % (uint64_t)Q4), j = (int)(rest / (uint64_t)Q4);
-        const float* __restrict__ row = X + (i * H + j) * W;
+        uint64_t mat = i;
+        bool ok = i < nmat;
+        if (ln) {
+            const uint64_t l = i / mp, t = i % mp;
+            mat = l + (uint64_t)ln * t;
+            ok = t < mpl;
+        }
+        const float* __restrict__ row = X + ((ok ? mat : 0) * H + j) * W;
         float v[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int q = 4 * p + t - PL;
-            v[t] = (q >= 0 && q < W) ? row[q] : 0.f;
+            v[t] = (ok && q >= 0 && q < W) ? row[q] : 0.f;
         }
         *reinterpret_cast<float4*>(Xq + e * 4) = make_float4(v[0], v[1], v[2], v[3]);
     }
@@ -290,32 +305,45 @@ __global__ __launch_bounds__(256) void pairs_copy_right_kernel(const float* __re
 
 constexpr int kPairsTY = 4;
 
-// Whether the lanes-over-pairs kernel serves this batch: 64 consecutive pairs share a left.
-bool pairs_kernel_applies(const family_args& a) {
-    const bool shared_left = a.family == XC_ONE_TO_MANY || (a.family == XC_N_TO_M && a.m % 64 == 0) ||
-                             (a.family == XC_N_TO_MN && a.n == 1);
-    return shared_left && a.pairs >= 64;
+// Rights per left of a family (0: one_to_one, which the kernel does not serve).
+uint32_t rights_per_left(const family_args& a) {
+    switch (a.family) {
+    case XC_ONE_TO_MANY: return a.m;
+    case XC_N_TO_M: return a.m;
+    case XC_N_TO_MN: return a.m / a.n;
+    default: return 0;
+    }
 }
+
+// Whether the lanes-over-pairs kernel serves this batch by default: at least 48 rights per
+// left (a wave's lanes at least 3/4 busy below 64).
+bool pairs_kernel_applies(const family_args& a) { return rights_per_left(a) >= 48; }
 
 // Waves of the lanes-over-pairs kernel.  Measured against the register-tile kernel (one_to_many
 // and n_to_m, 16^2 to 256^2): 0.45-0.9x at 128-1024 waves, 1.14x at 2048, 1.6-2.2x from 4096.
 uint64_t pairs_kernel_waves(const family_args& a) {
-    return (uint64_t)((a.pairs + 63) / 64) * ((2 * a.rows - 1 + kPairsTY - 1) / kPairsTY) * ((2 * a.cols - 1 + 3) / 4);
+    const uint32_t nl = a.family == XC_ONE_TO_MANY ? 1 : a.n;
+    return (uint64_t)nl * ((rights_per_left(a) + 63) / 64) * ((2 * a.rows - 1 + kPairsTY - 1) / kPairsTY) *
+           ((2 * a.cols - 1 + 3) / 4);
 }
 
 void launch_direct_pairs(const family_args& a, const float* L, const float* R, float* O, hipStream_t s) {
     constexpr int TY = kPairsTY;
     const int H = (int)a.rows, W = (int)a.cols;
-    const uint32_t nl = a.family == XC_N_TO_M ? a.n : 1;
-    const uint32_t nr = a.m;
+    const uint32_t nl = a.family == XC_ONE_TO_MANY ? 1 : a.n;
     dp_args k{};
     k.family = a.family;
     k.H = H;
     k.W = W;
-    k.n = a.family == XC_ONE_TO_MANY ? 1 : a.n;
+    k.n = nl;
     k.m = a.m;
     k.pairs = a.pairs;
-    k.groups = (a.pairs + 63) / 64;
+    k.mpl = rights_per_left(a);
+    k.wpl = (k.mpl + 63) / 64;
+    k.groups = nl * k.wpl;
+    // right slots: the rights in order, padded to a multiple of 64; n_to_mn: per left, its
+    // rights, padded to a multiple of 64 each
+    const uint32_t nr = a.family == XC_N_TO_MN ? nl * k.wpl * 64 : k.wpl * 64;
     k.tiles_y = (uint32_t)(2 * H - 1 + TY - 1) / TY;
     k.tiles_x = (uint32_t)(2 * W - 1 + 3) / 4;
     k.nbx = (k.tiles_x + 3) / 4;
@@ -332,8 +360,8 @@ void launch_direct_pairs(const family_args& a, const float* L, const float* R, f
     hipLaunchKernelGGL(pairs_copy_left_kernel, dim3(blocks_for(lsize)), dim3(256), 0, s, L, Lc, nl, H, W, k.Hl, k.Wl,
                        TY - 1);
     XC_HIP(hipGetLastError());
-    hipLaunchKernelGGL(pairs_copy_right_kernel, dim3(blocks_for(rsize / 4)), dim3(256), 0, s, R, Rq, nr, H, W, k.Q4,
-                       k.PL);
+    hipLaunchKernelGGL(pairs_copy_right_kernel, dim3(blocks_for(rsize / 4)), dim3(256), 0, s, R, Rq, nr, a.m,
+                       a.family == XC_N_TO_MN ? nl : 0u, k.wpl * 64, k.mpl, H, W, k.Q4, k.PL);
     XC_HIP(hipGetLastError());
     k.by_unit = rsize * sizeof(float) > (4u << 20);  // larger than one XCD's L2
     const uint64_t nblocks = blocks_for_order(k.groups * k.nbx, k.tiles_y, k.by_unit);
@@ -355,8 +383,8 @@ void launch_direct(const family_args& a, const T* L, const T* R, T* O, const xc_
         if (opt.tile > 2) fail(XC_ERR_INVALID_ARGUMENT, "FP32 tile must be 0 (default), 1 (register tiles) or 2 (lanes over pairs)");
         const bool keys = opt.shifts_per_thread || opt.right_mats_per_thread || opt.left_mats_per_thread ||
                           opt.rows_per_step || opt.waves_per_block || opt.split || opt.rows_per_block;
-        if (opt.tile == 2 && !pairs_kernel_applies(a))
-            fail(XC_ERR_UNSUPPORTED, "FP32 tile 2 needs 64 or more pairs whose consecutive 64 share a left matrix");
+        if (opt.tile == 2 && rights_per_left(a) == 0)
+            fail(XC_ERR_UNSUPPORTED, "FP32 tile 2 (lanes over pairs) does not serve one_to_one");
         if (opt.tile == 2 || (opt.tile == 0 && !keys && pairs_kernel_applies(a) && pairs_kernel_waves(a) >= 2048)) {
             launch_direct_pairs(a, L, R, O, s);
             return;

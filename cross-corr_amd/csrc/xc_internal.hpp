@@ -111,6 +111,9 @@ __device__ inline float4 buffer_quad(const float* base, uint32_t lane_bytes, uin
 __host__ __device__ inline uint64_t blocks_for_order(uint32_t units, uint32_t bands, bool by_unit) {
     return by_unit ? ((uint64_t)units + 7) / 8 * 8 * bands : (uint64_t)units * ((bands + 7) / 8 * 8);
 }
+// The results are the same for every lane (bid = blockIdx.x); readfirstlane says so to the
+// compiler, whose uniformity analysis otherwise loses them here and turns the callers' scalar
+// loads into vector loads (measured: 8.9 -> 11.2 ms for the FP32 pairs kernel at 64 x 256^2).
 __device__ inline bool block_order(uint32_t bid, uint32_t units, uint32_t bands, bool by_unit, uint32_t& unit,
                                    uint32_t& band) {
     const uint32_t xcd = bid % 8, pos = bid / 8;
@@ -118,11 +121,13 @@ __device__ inline bool block_order(uint32_t bid, uint32_t units, uint32_t bands,
         const uint32_t round = pos / bands;
         unit = round * 8 + ((round & 1) ? 7 - xcd : xcd);
         band = pos % bands;
-        return unit < units;
+    } else {
+        band = pos / units * 8 + xcd;
+        unit = pos % units;
     }
-    band = pos / units * 8 + xcd;
-    unit = pos % units;
-    return band < bands;
+    unit = __builtin_amdgcn_readfirstlane(unit);
+    band = __builtin_amdgcn_readfirstlane(band);
+    return by_unit ? unit < units : band < bands;
 }
 
 // Rank k of count in nearest-the-centre-first order: c, c+1, c-1, c+2, ... (c = (count-1)/2).

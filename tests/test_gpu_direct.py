@@ -160,13 +160,16 @@ def test_direct_triangle_distribution(rows_per_block):
             check("n_to_m", left, right, got, tol)
 
 
-# The lanes-over-pairs kernel (FP32 tile 2, and the default where every 64 consecutive pairs
-# share their left matrix and the batch makes enough waves): one pair per lane, 64 pairs per
-# wave, a partial last group of pairs, outputs narrower and wider than a tile, one row or
+# The lanes-over-pairs kernel (FP32 tile 2, and the default where a left has 48 or more rights
+# and the batch makes enough waves): a wave takes one left and 64 of its rights, one pair per
+# lane; partial last waves of a left, several lefts (n_to_m, and n_to_mn whose rights of a
+# left are n apart), few rights per left, outputs narrower and wider than a tile, one row or
 # column.
 PAIRS_SHAPES = [("one_to_many", 1, 1, 1, 64), ("one_to_many", 7, 5, 1, 70), ("one_to_many", 33, 65, 1, 128),
                 ("n_to_m", 17, 23, 2, 64), ("n_to_mn", 12, 30, 1, 65), ("one_to_many", 3, 200, 1, 64),
-                ("one_to_many", 150, 9, 1, 64), ("one_to_many", 1, 37, 1, 66), ("n_to_m", 40, 1, 3, 128)]
+                ("one_to_many", 150, 9, 1, 64), ("one_to_many", 1, 37, 1, 66), ("n_to_m", 40, 1, 3, 128),
+                ("n_to_m", 19, 21, 3, 100), ("n_to_mn", 14, 18, 3, 210), ("n_to_mn", 9, 40, 2, 128),
+                ("n_to_m", 5, 6, 2, 7), ("n_to_mn", 6, 5, 4, 8)]
 
 
 @pytest.mark.parametrize("family,rows,cols,n,m", PAIRS_SHAPES)
@@ -185,19 +188,27 @@ def test_direct_lanes_over_pairs(family, rows, cols, n, m):
     check(family, left, right, tiles, 1e-6)
 
 
-def test_direct_lanes_over_pairs_refused_where_it_does_not_apply():
+def test_direct_lanes_over_pairs_default_and_refusal():
     x = xcorr()
-    for family, n, m in (("one_to_many", 1, 63), ("n_to_m", 2, 32), ("n_to_mn", 2, 128), ("one_to_one", 1, 1)):
-        left = dev(seeded((n, 9, 9), "u01", 1, np.float32))
-        right = dev(seeded((m, 9, 9), "u01", 2, np.float32))
-        with pytest.raises(x.XcError) as e:
-            x.cross_corr(family, left, right, precision="fp32", tile=2)
-        assert e.value.status == 4
-        x.cross_corr(family, left, right, precision="fp32")  # the register-tile kernel serves it
+    # small batches: the default is the register-tile kernel
+    for family, n, m in (("one_to_many", 1, 63), ("n_to_m", 2, 32), ("n_to_mn", 2, 128)):
+        left = seeded((n, 9, 9), "u01", 1, np.float32)
+        right = seeded((m, 9, 9), "u01", 2, np.float32)
+        check(family, left, right, run(family, left, right, precision="fp32"), 1e-6)
         assert x.last_launch()["kernel"].startswith("direct_kernel")
+    # one_to_one has one right per left: tile 2 is refused
+    one = dev(seeded((1, 9, 9), "u01", 1, np.float32))
     with pytest.raises(x.XcError) as e:
-        x.cross_corr("one_to_one", left, right, precision="fp32", tile=3)
+        x.cross_corr("one_to_one", one, one, precision="fp32", tile=2)
+    assert e.value.status == 4
+    with pytest.raises(x.XcError) as e:
+        x.cross_corr("one_to_one", one, one, precision="fp32", tile=3)
     assert e.value.status == 1
+    # a large batch with 64 rights per left: the default is the lanes-over-pairs kernel
+    left = seeded((2, 64, 64), "u11", 3, np.float32)
+    right = seeded((64, 64, 64), "u11", 4, np.float32)
+    check("n_to_m", left, right, run("n_to_m", left, right, precision="fp32"), 1e-6)
+    assert x.last_launch()["kernel"].startswith("direct_pairs_kernel")
 
 
 def test_direct_split_and_triangle_are_exclusive():
