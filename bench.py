@@ -50,7 +50,10 @@ HBM_GBPS = 8000.0
 # direct kernel does not use packed FP32 instructions: half the published 157.3 TFLOP/s),
 # FP64 VALU, dense BF16/FP16 MFMA.
 TFLOPS = {"direct_kernel": 78.6, "direct_kernel_f64": 78.6, "mfma_kernel": 2500.0, "batch_kernel": 2500.0}
-METHODS = {"fp32": dict(precision="fp32"), "bf16x3": dict(precision="bf16x3"), "fp16x3": dict(precision="fp16x3"),
+# fp32: the library's FP32 default (the lanes-over-pairs kernel where it applies); fp32tiles: the
+# register-tile kernel (FP32 tile 1), for comparison runs only.
+METHODS = {"fp32": dict(precision="fp32"), "fp32tiles": dict(precision="fp32", tile=1),
+           "bf16x3": dict(precision="bf16x3"), "fp16x3": dict(precision="fp16x3"),
            "bf16": dict(precision="bf16"), "fp16": dict(precision="fp16"),
            "fft64": dict(precision="fft64", tile=0), "fft64k": dict(precision="fft64", tile=1),
            "auto": dict(precision="auto")}
@@ -245,7 +248,7 @@ def main():
             results[method] = {"error": str(e)}
             continue
         err, _ = norm_error(out[picks].cpu().numpy(), ref, den)
-        steps = args.steps if method not in ("fp32",) else max(3, args.steps // 4)
+        steps = args.steps if method not in ("fp32", "fp32tiles") else max(3, args.steps // 4)
         ms = timed(lambda: step(method), steps, args.warmup)
         results[method] = {"ms_per_step": ms, "pairs_per_s": NUM_RIGHT / (ms * 1e-3),
                            "gmac_per_s": NUM_RIGHT * macs / (ms * 1e-3) / 1e9,
