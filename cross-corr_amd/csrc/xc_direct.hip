@@ -162,10 +162,31 @@ struct dp_args {
     int by_unit;                      // block order (see the kernel)
 };
 
-template <int TY>
-__global__ __launch_bounds__(256) void direct_pairs_kernel(const float* __restrict__ Lc, const float* __restrict__ Rq,
-                                                           float* __restrict__ O, dp_args a) {
-    constexpr int TX = 4, FLUSHC = 16;  // FLUSHC chunks of 4 columns per FP32 partial
+// Four consecutive elements of a row: 16 bytes (float) or 32 bytes (double).
+template <typename T>
+struct quad_of;
+template <>
+struct quad_of<float> {
+    using type = float4;
+    // the lane's quad of a right-copy quad position (base wave-uniform)
+    __device__ static float4 load(const float* base, uint32_t lane_bytes) { return buffer_quad(base, lane_bytes, 0); }
+};
+template <>
+struct quad_of<double> {
+    using type = double4;
+    __device__ static double4 load(const double* base, uint32_t lane_bytes) {
+        const float* b = reinterpret_cast<const float*>(base);
+        const double2 lo = __builtin_bit_cast(double2, buffer_quad(b, lane_bytes, 0));
+        const double2 hi = __builtin_bit_cast(double2, buffer_quad(b, lane_bytes, 16));
+        return make_double4(lo.x, lo.y, hi.x, hi.y);
+    }
+};
+
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void direct_pairs_kernel(const T* __restrict__ Lc, const T* __restrict__ Rq,
+                                                           T* __restrict__ O, dp_args a) {
+    using Q = typename quad_of<T>::type;
+    constexpr int TX = 4, FLUSHC = 16;  // FLUSHC chunks of 4 columns per partial sum
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     // Blocks go round-robin to the 8 XCDs; tiles run nearest the centre (largest overlap)
@@ -194,16 +215,16 @@ __global__ __launch_bounds__(256) void direct_pairs_kernel(const float* __restri
     const int jlo = max(0, -(dy0 + TY - 1)), jhi = min(H, H - dy0);
     const int clo = max(0, dx0) & ~3;
     const int nch = (min(W, W + dx0 + TX - 1) - clo + 3) >> 2;
-    const uint32_t lane_bytes = slot * 16u;
-    const int64_t rq = (int64_t)a.r_count * 4;  // floats between quad positions of one row
-    const float* __restrict__ lt = Lc + ((int64_t)li * a.Hl + dy0 + TY - 1) * a.Wl + clo;  // + j * Wl: row j + dy0
-    const float* __restrict__ rt = Rq + (int64_t)((clo - dx0 - 3 + a.PL) >> 2) * rq;       // + j * Q4 * rq: row j
-    float acc[TY][TX];
+    const uint32_t lane_bytes = slot * 4u * (uint32_t)sizeof(T);
+    const int64_t rq = (int64_t)a.r_count * 4;  // elements between quad positions of one row
+    const T* __restrict__ lt = Lc + ((int64_t)li * a.Hl + dy0 + TY - 1) * a.Wl + clo;  // + j * Wl: row j + dy0
+    const T* __restrict__ rt = Rq + (int64_t)((clo - dx0 - 3 + a.PL) >> 2) * rq;       // + j * Q4 * rq: row j
+    T acc[TY][TX];
     double tot[TY][TX];
 #pragma unroll
     for (int y = 0; y < TY; ++y)
 #pragma unroll
-        for (int x = 0; x < TX; ++x) acc[y][x] = 0.f, tot[y][x] = 0.0;
+        for (int x = 0; x < TX; ++x) acc[y][x] = T(0), tot[y][x] = 0.0;
     int cnt = 0;
     auto flush = [&]() {
 #pragma unroll
@@ -211,31 +232,31 @@ __global__ __launch_bounds__(256) void direct_pairs_kernel(const float* __restri
 #pragma unroll
             for (int x = 0; x < TX; ++x) {
                 tot[y][x] += (double)acc[y][x];
-                acc[y][x] = 0.f;
+                acc[y][x] = T(0);
             }
     };
     for (int j = jlo; j < jhi; ++j) {
-        const float* __restrict__ lr = lt + (int64_t)j * a.Wl;
-        const float* __restrict__ rr = rt + (int64_t)j * a.Q4 * rq;
-        float4 w0 = buffer_quad(rr, lane_bytes, 0), w1 = buffer_quad(rr + rq, lane_bytes, 0);
-        float4 lv[TY];
+        const T* __restrict__ lr = lt + (int64_t)j * a.Wl;
+        const T* __restrict__ rr = rt + (int64_t)j * a.Q4 * rq;
+        Q w0 = quad_of<T>::load(rr, lane_bytes), w1 = quad_of<T>::load(rr + rq, lane_bytes);
+        Q lv[TY];
 #pragma unroll
-        for (int y = 0; y < TY; ++y) lv[y] = *reinterpret_cast<const float4*>(lr + (int64_t)y * a.Wl);
+        for (int y = 0; y < TY; ++y) lv[y] = *reinterpret_cast<const Q*>(lr + (int64_t)y * a.Wl);
         for (int ch = 0; ch < nch; ++ch) {
             // the next chunk's values in flight during this one (the copies hold a quad past
             // the last window, the left copy a chunk past the last one)
-            const float4 w2 = buffer_quad(rr + (ch + 2) * rq, lane_bytes, 0);
-            float4 ln[TY];
+            const Q w2 = quad_of<T>::load(rr + (ch + 2) * rq, lane_bytes);
+            Q ln[TY];
 #pragma unroll
-            for (int y = 0; y < TY; ++y) ln[y] = *reinterpret_cast<const float4*>(lr + (int64_t)y * a.Wl + 4 * ch + 4);
-            const float w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+            for (int y = 0; y < TY; ++y) ln[y] = *reinterpret_cast<const Q*>(lr + (int64_t)y * a.Wl + 4 * ch + 4);
+            const T w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
 #pragma unroll
             for (int y = 0; y < TY; ++y) {
-                const float l[4] = {lv[y].x, lv[y].y, lv[y].z, lv[y].w};
+                const T l[4] = {lv[y].x, lv[y].y, lv[y].z, lv[y].w};
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
 #pragma unroll
-                    for (int x = 0; x < TX; ++x) acc[y][x] = fmaf(l[t], w[t + 3 - x], acc[y][x]);
+                    for (int x = 0; x < TX; ++x) acc[y][x] = fma(l[t], w[t + 3 - x], acc[y][x]);
             }
             w0 = w1;
             w1 = w2;
@@ -249,20 +270,21 @@ __global__ __launch_bounds__(256) void direct_pairs_kernel(const float* __restri
     }
     flush();
     if (!valid) return;
-    float* __restrict__ op = O + (size_t)pk * (2 * W - 1) * (2 * H - 1);
+    T* __restrict__ op = O + (size_t)pk * (2 * W - 1) * (2 * H - 1);
 #pragma unroll
     for (int y = 0; y < TY; ++y) {
         const int dy = dy0 + y;
         if (dy > H - 1) break;
 #pragma unroll
         for (int x = 0; x < TX; ++x)
-            if (dx0 + x <= W - 1) op[(size_t)(dy + H - 1) * (2 * W - 1) + (dx0 + x + W - 1)] = (float)tot[y][x];
+            if (dx0 + x <= W - 1) op[(size_t)(dy + H - 1) * (2 * W - 1) + (dx0 + x + W - 1)] = (T)tot[y][x];
     }
 }
 
 // Left copy: count matrices of H x W into rows of Wl (zero past W) with pad zero rows above
 // and below: element (r, c) of matrix i at (i * Hl + r + pad) * Wl + c.
-__global__ __launch_bounds__(256) void pairs_copy_left_kernel(const float* __restrict__ X, float* __restrict__ Xc,
+template <typename T>
+__global__ __launch_bounds__(256) void pairs_copy_left_kernel(const T* __restrict__ X, T* __restrict__ Xc,
                                                               uint32_t count, int H, int W, int Hl, int Wl, int pad) {
     const uint64_t total = (uint64_t)count * Hl * Wl;
     for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
@@ -270,7 +292,7 @@ __global__ __launch_bounds__(256) void pairs_copy_left_kernel(const float* __res
         const uint64_t rest = e / (uint64_t)Wl;
         const int r = (int)(rest % (uint64_t)Hl) - pad;
         const uint64_t i = rest / (uint64_t)Hl;
-        Xc[e] = (r >= 0 && r < H && c < W) ? X[(i * H + r) * W + c] : 0.f;
+        Xc[e] = (r >= 0 && r < H && c < W) ? X[(i * H + r) * W + c] : T(0);
     }
 }
 
@@ -278,7 +300,8 @@ __global__ __launch_bounds__(256) void pairs_copy_left_kernel(const float* __res
 // columns 4p - PL .. 4p - PL + 3 (zero outside [0, W)) of the slot's matrix: with ln = 0 slot
 // i is matrix i (zero for i >= nmat); with ln lefts (n_to_mn), slot l*MP + t is matrix
 // l + ln*t (zero for t >= mpl).  Thread e writes quad e.
-__global__ __launch_bounds__(256) void pairs_copy_right_kernel(const float* __restrict__ X, float* __restrict__ Xq,
+template <typename T>
+__global__ __launch_bounds__(256) void pairs_copy_right_kernel(const T* __restrict__ X, T* __restrict__ Xq,
                                                                uint32_t count, uint32_t nmat, uint32_t ln,
                                                                uint32_t mp, uint32_t mpl, int H, int W, int Q4, int PL) {
     const uint64_t total = (uint64_t)count * H * Q4;
@@ -292,14 +315,12 @@ __global__ __launch_bounds__(256) void pairs_copy_right_kernel(const float* __re
             mat = l + (uint64_t)ln * t;
             ok = t < mpl;
         }
-        const float* __restrict__ row = X + ((ok ? mat : 0) * H + j) * W;
-        float v[4];
+        const T* __restrict__ row = X + ((ok ? mat : 0) * H + j) * W;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int q = 4 * p + t - PL;
-            v[t] = (ok && q >= 0 && q < W) ? row[q] : 0.f;
+            Xq[e * 4 + t] = (ok && q >= 0 && q < W) ? row[q] : T(0);
         }
-        *reinterpret_cast<float4*>(Xq + e * 4) = make_float4(v[0], v[1], v[2], v[3]);
     }
 }
 
@@ -316,8 +337,16 @@ uint32_t rights_per_left(const family_args& a) {
 }
 
 // Whether the lanes-over-pairs kernel serves this batch by default: at least 48 rights per
-// left (a wave's lanes at least 3/4 busy below 64).
-bool pairs_kernel_applies(const family_args& a) { return rights_per_left(a) >= 48; }
+// left (a wave's lanes at least 3/4 busy below 64), and its copy of the rights (the register-
+// tile kernel needs none) at most 16 GiB.
+bool pairs_kernel_applies(const family_args& a) {
+    const uint32_t mpl = rights_per_left(a);
+    if (mpl < 48) return false;
+    const uint64_t lefts = a.family == XC_ONE_TO_MANY ? 1 : a.n;
+    const uint64_t slots = (a.family == XC_N_TO_MN ? lefts : 1) * ((mpl + 63) / 64 * 64);
+    const uint64_t copy = slots * a.rows * (a.cols + 24) * sizeof(float);
+    return copy <= (16ull << 30);
+}
 
 // Waves of the lanes-over-pairs kernel.  Measured against the register-tile kernel (one_to_many
 // and n_to_m, 16^2 to 256^2): 0.45-0.9x at 128-1024 waves, 1.14x at 2048, 1.6-2.2x from 4096.
@@ -327,7 +356,8 @@ uint64_t pairs_kernel_waves(const family_args& a) {
            ((2 * a.cols - 1 + 3) / 4);
 }
 
-void launch_direct_pairs(const family_args& a, const float* L, const float* R, float* O, hipStream_t s) {
+template <typename T>
+void launch_direct_pairs(const family_args& a, const T* L, const T* R, T* O, hipStream_t s) {
     constexpr int TY = kPairsTY;
     const int H = (int)a.rows, W = (int)a.cols;
     const uint32_t nl = a.family == XC_ONE_TO_MANY ? 1 : a.n;
@@ -352,43 +382,43 @@ void launch_direct_pairs(const family_args& a, const float* L, const float* R, f
     k.Q4 = (k.PL + W + 12 + 3) / 4;  // a quad past the last window
     k.Hl = H + 2 * (TY - 1);
     k.Wl = (W + 3) / 4 * 4 + 4;       // a chunk past the last one (prefetch)
-    if ((uint64_t)nr * 16 >= (1ull << 31)) fail(XC_ERR_UNSUPPORTED, "direct kernel: too many right matrices");
+    if ((uint64_t)nr * 4 * sizeof(T) + 16 >= (1ull << 31)) fail(XC_ERR_UNSUPPORTED, "direct kernel: too many right matrices");
     const uint64_t lsize = (uint64_t)nl * k.Hl * k.Wl, rsize = (uint64_t)nr * H * k.Q4 * 4;
-    float* Lc = static_cast<float*>(workspace(WS_DIRECT, sizeof(float) * (lsize + rsize), s));
-    float* Rq = Lc + lsize;
+    T* Lc = static_cast<T*>(workspace(WS_DIRECT, sizeof(T) * (lsize + rsize), s));
+    T* Rq = Lc + lsize;
     auto blocks_for = [](uint64_t n) { return (uint32_t)std::min<uint64_t>((n + 255) / 256, 16384); };
-    hipLaunchKernelGGL(pairs_copy_left_kernel, dim3(blocks_for(lsize)), dim3(256), 0, s, L, Lc, nl, H, W, k.Hl, k.Wl,
-                       TY - 1);
+    hipLaunchKernelGGL(pairs_copy_left_kernel<T>, dim3(blocks_for(lsize)), dim3(256), 0, s, L, Lc, nl, H, W, k.Hl,
+                       k.Wl, TY - 1);
     XC_HIP(hipGetLastError());
-    hipLaunchKernelGGL(pairs_copy_right_kernel, dim3(blocks_for(rsize / 4)), dim3(256), 0, s, R, Rq, nr, a.m,
+    hipLaunchKernelGGL(pairs_copy_right_kernel<T>, dim3(blocks_for(rsize / 4)), dim3(256), 0, s, R, Rq, nr, a.m,
                        a.family == XC_N_TO_MN ? nl : 0u, k.wpl * 64, k.mpl, H, W, k.Q4, k.PL);
     XC_HIP(hipGetLastError());
-    k.by_unit = rsize * sizeof(float) > (4u << 20);  // larger than one XCD's L2
+    k.by_unit = rsize * sizeof(T) > (4u << 20);  // larger than one XCD's L2
     const uint64_t nblocks = blocks_for_order(k.groups * k.nbx, k.tiles_y, k.by_unit);
     if (nblocks > 0x7fffffffULL) fail(XC_ERR_INVALID_ARGUMENT, "problem too large for one launch");
     dim3 grid((uint32_t)nblocks), block(256);
-    record_launch("direct_pairs_kernel<f32> ty4 tx4", grid, block, 0, a.rows, a.cols, a.pairs, 1);
+    record_launch(sizeof(T) == 4 ? "direct_pairs_kernel<f32> ty4 tx4" : "direct_pairs_kernel<f64> ty4 tx4", grid, block,
+                  0, a.rows, a.cols, a.pairs, 1);
     timed td("direct_kernel", s);
-    hipLaunchKernelGGL(direct_pairs_kernel<TY>, grid, block, 0, s, Lc, Rq, O, k);
+    hipLaunchKernelGGL((direct_pairs_kernel<T, TY>), grid, block, 0, s, Lc, Rq, O, k);
     XC_HIP(hipGetLastError());
 }
 
 template <typename T>
 void launch_direct(const family_args& a, const T* L, const T* R, T* O, const xc_options& opt, hipStream_t s) {
     if (a.cols > (uint32_t)kMaxCols) fail(XC_ERR_UNSUPPORTED, "direct kernel: cols above 32768");
-    if constexpr (sizeof(T) == 4) {
-        // tile 0: the lanes-over-pairs kernel where it applies, the batch gives 2048 waves or
-        // more and no register tile or work distribution is requested; 1: the register-tile
-        // kernel; 2: the lanes-over-pairs kernel
-        if (opt.tile > 2) fail(XC_ERR_INVALID_ARGUMENT, "FP32 tile must be 0 (default), 1 (register tiles) or 2 (lanes over pairs)");
-        const bool keys = opt.shifts_per_thread || opt.right_mats_per_thread || opt.left_mats_per_thread ||
-                          opt.rows_per_step || opt.waves_per_block || opt.split || opt.rows_per_block;
-        if (opt.tile == 2 && rights_per_left(a) == 0)
-            fail(XC_ERR_UNSUPPORTED, "FP32 tile 2 (lanes over pairs) does not serve one_to_one");
-        if (opt.tile == 2 || (opt.tile == 0 && !keys && pairs_kernel_applies(a) && pairs_kernel_waves(a) >= 2048)) {
-            launch_direct_pairs(a, L, R, O, s);
-            return;
-        }
+    // tile 0: the lanes-over-pairs kernel where it applies, the batch gives 2048 waves or more
+    // and no register tile or work distribution is requested; 1: the register-tile kernel;
+    // 2: the lanes-over-pairs kernel (float and double data alike)
+    if (opt.tile > 2)
+        fail(XC_ERR_INVALID_ARGUMENT, "direct-method tile must be 0 (default), 1 (register tiles) or 2 (lanes over pairs)");
+    const bool keys = opt.shifts_per_thread || opt.right_mats_per_thread || opt.left_mats_per_thread ||
+                      opt.rows_per_step || opt.waves_per_block || opt.split || opt.rows_per_block;
+    if (opt.tile == 2 && rights_per_left(a) == 0)
+        fail(XC_ERR_UNSUPPORTED, "direct-method tile 2 (lanes over pairs) does not serve one_to_one");
+    if (opt.tile == 2 || (opt.tile == 0 && !keys && pairs_kernel_applies(a) && pairs_kernel_waves(a) >= 2048)) {
+        launch_direct_pairs<T>(a, L, R, O, s);
+        return;
     }
     direct_plan d = plan_direct<T>(a, opt);
     if (d.lds > 160 * 1024) {

@@ -88,8 +88,9 @@ typedef enum xc_precision {
 typedef struct xc_options {
     uint32_t precision;   /* xc_precision */
     uint32_t tile;        /* kernel-specific tile code from the tile catalog, 0 = default;
-                             FP32 (float data): 1 = register-tile kernel, 2 = lanes-over-pairs
-                             kernel (64 consecutive pairs sharing a left matrix) */
+                             FP32 (the direct method, float or double data): 1 = register-tile
+                             kernel, 2 = lanes-over-pairs kernel (a left and 64 of its rights
+                             per wave; not for one_to_one) */
     uint32_t split;       /* FP32: split of the reduction (overlap rows) across blocks;
                              FFT64 tile 1: at most this many pairs per launch of the column
                              and inverse-row kernels (less scratch memory); 0 = auto */

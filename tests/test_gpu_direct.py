@@ -188,6 +188,21 @@ def test_direct_lanes_over_pairs(family, rows, cols, n, m):
     check(family, left, right, tiles, 1e-6)
 
 
+@pytest.mark.parametrize("family,rows,cols,n,m", PAIRS_SHAPES[1:6] + PAIRS_SHAPES[9:12])
+def test_direct_lanes_over_pairs_double(family, rows, cols, n, m):
+    """The same kernel on double data (FP64 FMAs, the FP64 direct method's bar)."""
+    x = xcorr()
+    left = seeded((n, rows, cols), "normal", 990 + rows, np.float64)
+    right = seeded((m, rows, cols), "normal", 995 + cols, np.float64)
+    print(f"lanes over pairs f64 {family} {rows}x{cols} n={n} m={m}", flush=True)
+    got = run(family, left, right, precision="fp32", tile=2)
+    assert x.last_launch()["kernel"].startswith("direct_pairs_kernel<f64>")
+    check(family, left, right, got, 1e-12)
+    tiles = run(family, left, right, precision="fp32", tile=1)
+    assert x.last_launch()["kernel"].startswith("direct_kernel<f64>")
+    check(family, left, right, tiles, 1e-12)
+
+
 def test_direct_lanes_over_pairs_default_and_refusal():
     x = xcorr()
     # small batches: the default is the register-tile kernel
