@@ -274,6 +274,10 @@ def main():
     if "ms_per_step" in results.get("fp32", {}):
         t32 = kernels_of(lambda: step("fp32"), 2)
         fp32_roof = roofline(t32, lambda k: None, lambda k: 2.0 * NUM_RIGHT * macs)
+        # "direct_kernel" is the direct method's timer, shared by its two kernels: name the
+        # kernel that ran.
+        fp32_roof["timer"] = fp32_roof["kernel"]
+        fp32_roof["kernel"] = results["fp32"].get("last_kernel", fp32_roof["kernel"])
 
     # CPU baseline: the oracle's FP32 restatement of cpu_cross_corr_one_to_many
     # (src/validate.hpp:231-247), single thread, on a bounded sample of the same workload.
